@@ -1,0 +1,301 @@
+/*
+ * hr.h - C ABI of the MI355X hybrid pass (G-buffer-seeded shadow and
+ * reflection ray tracing plus deferred PBR shading) of VulkanHybridRaytracer.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an hr_status. Errors are returned, never aborted on; the message of the last
+ * failure is kept per context (hr_last_error).
+ *
+ * Pointer conventions
+ *   - Scene inputs (meshes, primitives, materials, environment maps) are HOST
+ *     pointers; the library copies them into device memory once, as the
+ *     reference uploads its vertex/index/material buffers through staging
+ *     buffers (vk_mesh.cpp:265-359, renderer.cpp:1477-1530).
+ *   - Per-frame uniforms (lights, camera, parameters) are HOST pointers to a
+ *     few hundred bytes; they travel as kernel arguments, so no copy is made.
+ *   - Images (G-buffer planes, shadow, lit, reflection buffers) and ray/hit
+ *     arrays are DEVICE pointers (hr_device_alloc or any HIP allocation on the
+ *     context's device). They replace the Vulkan storage images of
+ *     renderer.cpp:1532-1624.
+ *   - Handles are not thread-safe: use one context per host thread.
+ *   - All work of a context is ordered on the context's HIP stream; entry
+ *     points that return results to the host synchronise that stream.
+ *
+ * Reference interfaces replaced (citations into the reference tree):
+ *   hr_scene_build    create_bottom_acceleration_structure + buildBlas +
+ *                     create_top_acceleration_structure + buildTlas
+ *                     (renderer.cpp:1643-1688, 1696-1865), fed by
+ *                     Node::node_to_geometry / node_to_instance
+ *                     (vk_mesh.cpp:424-492) and the per-primitive id/matrix
+ *                     tables (vk_mesh.cpp:506-533, renderer.cpp:3018-3048).
+ *   hr_scene_refit    per-frame TLAS update (vk_engine.cpp:174-187,
+ *                     renderer.cpp:1757-1865 with update = true).
+ *   hr_gbuffer_primary  the raster G-buffer pass (renderer.cpp:1377-1434,
+ *                     geometry_shader.frag:51-76, skybox.frag:14-21), replaced
+ *                     by primary visibility at pixel centres with the camera
+ *                     convention of raygen.rgen:29-40.
+ *   hr_trace_shadows  shadowRaygen.rgen + shadow.rmiss + shadowHit.rchit
+ *                     dispatched by build_shadow_command_buffer
+ *                     (renderer.cpp:2618-2659), descriptor set of
+ *                     renderer.cpp:1919-2028.
+ *   hr_shade_hybrid   hybridRaygen.rgen + hybridHit.rchit + miss.rmiss +
+ *                     shadow.rmiss dispatched by build_hybrid_command_buffers
+ *                     (renderer.cpp:3131-3175), descriptor set of
+ *                     renderer.cpp:2917-3129.
+ *   hr_trace_rays     a bare traceRayEXT (hybridRaygen.rgen:158 closest hit,
+ *                     shadowRaygen.rgen:78 terminate-on-first-hit).
+ */
+#ifndef HR_HR_H
+#define HR_HR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HR_ABI_VERSION 1u
+#define HR_MAX_LIGHTS 16u
+
+typedef enum hr_status {
+    HR_OK = 0,
+    HR_E_INVALID = 1,     /* bad argument: null pointer, size, index out of range */
+    HR_E_OOM = 2,         /* device allocation failed */
+    HR_E_DEVICE = 3,      /* HIP runtime error */
+    HR_E_UNSUPPORTED = 4, /* valid request outside what this library implements */
+    HR_E_OVERFLOW = 5     /* a fixed-capacity structure (traversal stack, queue) overflowed */
+} hr_status;
+
+/* == Vertex (vk_mesh.h:21-27): 44 bytes, position at offset 0 read with stride
+ * 44 exactly as the BLAS input does (vk_mesh.cpp:436-442). */
+typedef struct hr_vertex {
+    float position[3];
+    float normal[3];
+    float color[3];
+    float uv[2];
+} hr_vertex;
+
+/* == GPUMaterial (material.h:5-10), filled as Material::materialToShader
+ * (material.cpp:19-26): diffuse = (rgb, ior); textures = texture indices as
+ * floats, -1 = none; shading_metal_rough_idx = (shadingModel, metallic,
+ * roughness, material index). Shading models 0 (PBR), 3 (mirror/glossy) and
+ * 4 (glass) exist (hybridRaygen.rgen:105-148). */
+typedef struct hr_material {
+    float diffuse_ior[4];
+    float textures[4];
+    float shading_metal_rough_idx[4];
+} hr_material;
+
+/* == uboLight (vk_engine.h:76-80), packed as vk_engine.cpp:146-164:
+ * pos_maxdist = (position, maxDistance) for point lights and
+ * (direction towards the light, -1) for directional lights;
+ * color_intensity = (rgb, intensity); radius = disk radius for soft shadows. */
+typedef struct hr_light {
+    float pos_maxdist[4];
+    float color_intensity[4];
+    float radius;
+    float _pad[3];
+} hr_light;
+
+/* Optional per-light shape (no counterpart in the reference, whose lights are
+ * disks sampled by helpers.glsl:82-93). type HR_LIGHT_DISK keeps that
+ * behaviour; HR_LIGHT_RECT is a rectangle centred at pos_maxdist.xyz spanned by
+ * edge_u and edge_v (full extents), sampled on a stratified grid. */
+#define HR_LIGHT_DISK 0u
+#define HR_LIGHT_RECT 1u
+typedef struct hr_light_shape {
+    uint32_t type;
+    float edge_u[3];
+    float edge_v[3];
+    uint32_t _pad;
+} hr_light_shape;
+
+/* == RTCameraData (vk_engine.h:56-62), column-major glm matrices, filled as
+ * vk_engine.cpp:730-740: inverse(view), inverse(projection with [1][1] *= -1),
+ * frame = (frame index, ...). frame[0] seeds the per-pixel random streams
+ * through uint(int(frame)) (hybridRaygen.rgen:29). */
+typedef struct hr_camera {
+    float inv_view[16];
+    float inv_proj[16];
+    float frame[4];
+} hr_camera;
+
+/* One mesh: its vertex array and uint32 index array (Mesh::_vertices,
+ * Mesh::_indices, vk_mesh.h:73-77). */
+typedef struct hr_mesh {
+    const hr_vertex* vertices;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count;
+} hr_mesh;
+
+/* One drawable primitive (Primitive, vk_mesh.h:60-70) placed in the world by
+ * model = entity.m_matrix * node.global (vk_mesh.cpp:476, column-major).
+ * Its triangles are indices[first_index .. first_index + index_count). */
+typedef struct hr_primitive {
+    uint32_t mesh;
+    uint32_t first_index;
+    uint32_t index_count;
+    int32_t material;
+    float model[16];
+} hr_primitive;
+
+/* Environment maps, RGBA float texels, row-major, row 0 at v = 0.
+ * skybox = environmentTexture[0] / skybox[0] (miss.rmiss:15-17),
+ * irradiance = environmentTexture[1] (hybridRaygen.rgen:59-60,
+ * hybridHit.rchit:71-73). Looked up at the nearest texel with wrap in u and v
+ * (NEAREST/REPEAT sampler, renderer.cpp:3060, vk_initializers.h:86). */
+typedef struct hr_env {
+    const float* skybox;
+    uint32_t sky_width, sky_height;
+    const float* irradiance;
+    uint32_t irr_width, irr_height;
+} hr_env;
+
+/* FP32 G-buffer, one float4 per pixel per plane, row-major, device pointers.
+ *   position = (world xyz, material index)          [gbuffers[0]]
+ *   normal   = (world unit normal xyz, 1; 0 for background) [gbuffers[1]]
+ *   albedo   = (diffuse rgb before linearisation, 1) [gbuffers[2]]
+ *   material = (0, roughness, metallic, 0)           [gbuffers[4]]
+ *   emissive = (rgb, 0)                              [gbuffers[5]]
+ * Background pixels have position.w == 0 and normal.w == 0 and the sky colour
+ * in albedo (skybox.frag:16-21). The normal is stored decoded (the reference
+ * stores 0.5 n + 0.5 in RGBA16F, geometry_shader.frag:67). */
+typedef struct hr_gbuffer {
+    uint32_t width, height;
+    float* position;
+    float* normal;
+    float* albedo;
+    float* material;
+    float* emissive;
+} hr_gbuffer;
+
+/* Trace parameters.
+ * shadow_samples: shadow rays per light per pixel (UBO `samples`,
+ *   vk_engine.cpp:167-170, GUI range 1..64, renderer.cpp:656).
+ * refl_spp: reflection/refraction paths per shading-model 3/4 pixel
+ *   (1 in the reference).
+ * max_depth: bounce limit (MAX_RECURSION = 8, helpers.glsl:8). */
+typedef struct hr_trace_params {
+    uint32_t shadow_samples;
+    uint32_t refl_spp;
+    uint32_t max_depth;
+    uint32_t flags;
+} hr_trace_params;
+
+/* A ray for hr_trace_rays: 32 bytes, as traceRayEXT's (origin, tmin,
+ * direction, tmax) arguments. */
+typedef struct hr_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} hr_ray;
+
+/* A hit: 16 bytes. t = gl_HitTEXT or -1 on a miss; (u, v) = the barycentric
+ * hit attributes (weights of the triangle's 2nd and 3rd vertex,
+ * hybridHit.rchit:27); tri = global triangle id (primitive order, then the
+ * primitive's triangle order; 0xFFFFFFFF on a miss). For any-hit queries only
+ * t (>= 0 means occluded) and tri are meaningful. */
+typedef struct hr_hit {
+    float t;
+    float u;
+    float v;
+    uint32_t tri;
+} hr_hit;
+
+/* Ray counts and timings of the last frame call. */
+typedef struct hr_stats {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;      /* G-buffer-surface shadow rays (shadowRaygen) */
+    uint64_t reflection_rays;  /* closest-hit bounce rays (hybridRaygen:158) */
+    uint64_t hit_shadow_rays;  /* shadow rays cast at bounce hits (hybridHit:123) */
+    uint64_t occluded_rays;    /* shadow rays of both kinds that found a hit */
+    uint64_t node_visits;      /* only with HR_FLAG_COUNT_VISITS */
+    uint64_t tri_tests;        /* only with HR_FLAG_COUNT_VISITS */
+    float ms_gbuffer;
+    float ms_shadow;
+    float ms_shade;
+    float ms_build;
+} hr_stats;
+
+#define HR_FLAG_COUNT_VISITS 1u  /* count node visits / triangle tests (slower kernel variant) */
+#define HR_FLAG_TIMING 2u        /* record HIP events around each kernel */
+
+typedef struct hr_build_params {
+    uint32_t leaf_max;   /* largest Karras subtree stored as one leaf (1..8, default 4) */
+    uint32_t _reserved[3];
+} hr_build_params;
+
+typedef struct hr_context hr_context;
+typedef struct hr_scene hr_scene;
+
+/* ---- context ---------------------------------------------------------- */
+uint32_t hr_abi_version(void);
+hr_status hr_context_create(int device, hr_context** out);
+void hr_context_destroy(hr_context* ctx);
+const char* hr_last_error(const hr_context* ctx);
+hr_status hr_synchronize(hr_context* ctx);
+
+/* Device memory helpers for hosts without their own allocator. */
+hr_status hr_device_alloc(hr_context* ctx, size_t bytes, void** out);
+hr_status hr_device_free(hr_context* ctx, void* ptr);
+hr_status hr_memcpy_h2d(hr_context* ctx, void* dst_dev, const void* src_host, size_t bytes);
+hr_status hr_memcpy_d2h(hr_context* ctx, void* dst_host, const void* src_dev, size_t bytes);
+hr_status hr_memset_d(hr_context* ctx, void* dst_dev, int value, size_t bytes);
+
+/* ---- scene (replaces the BLAS/TLAS build, renderer.cpp:1643-1916) ------- */
+hr_status hr_scene_build(hr_context* ctx,
+                         const hr_mesh* meshes, uint32_t mesh_count,
+                         const hr_primitive* prims, uint32_t prim_count,
+                         const hr_material* materials, uint32_t material_count,
+                         const hr_build_params* params, /* may be NULL */
+                         hr_scene** out);
+/* New model matrices (prim_count x 16 floats, host): re-transform triangles and
+ * refit node boxes on the existing topology (TLAS update,
+ * vk_engine.cpp:174-187). */
+hr_status hr_scene_refit(hr_scene* scene, const float* models);
+/* Material re-upload (renderer.cpp:761-774). */
+hr_status hr_scene_set_materials(hr_scene* scene, const hr_material* materials, uint32_t count);
+hr_status hr_scene_set_env(hr_scene* scene, const hr_env* env);
+void hr_scene_destroy(hr_scene* scene);
+
+/* Scene layout queries and device-array export (for validation). */
+hr_status hr_scene_info(const hr_scene* scene, uint32_t* tri_count, uint32_t* node_count,
+                        uint32_t* depth, float* ms_build);
+/* which: 0 = traversal nodes (node_count x 64 B), 1 = sorted world-space
+ * triangles (tri_count x 48 B), 2 = Morton codes in sorted order (tri_count x
+ * 4 B), 3 = sorted triangle ids (tri_count x 4 B), 4 = Karras internal nodes
+ * (left, right, first, last) ((tri_count - 1) x 16 B). */
+hr_status hr_scene_export(const hr_scene* scene, uint32_t which, void* dst_host, size_t bytes);
+
+/* ---- per-frame passes ---------------------------------------------------- */
+hr_status hr_gbuffer_primary(hr_scene* scene, const hr_camera* cam, const hr_gbuffer* gb,
+                             hr_stats* stats);
+
+/* shadow: device float[nl][H][W], fraction of unoccluded samples per light. */
+hr_status hr_trace_shadows(hr_scene* scene, const hr_gbuffer* gb,
+                           const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                           const hr_camera* cam, const hr_trace_params* params,
+                           float* shadow, hr_stats* stats);
+
+/* lit: device float4[H][W] final colour (imageStore, hybridRaygen.rgen:186,
+ * unquantised); refl: device float4[H][W], the multiplicative bounce factor
+ * averaged over refl_spp paths (1 where nothing is traced), so that
+ * lit = direct * refl + ambient + emissive for non-background pixels. */
+hr_status hr_shade_hybrid(hr_scene* scene, const hr_gbuffer* gb,
+                          const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                          const hr_camera* cam, const hr_trace_params* params,
+                          const float* shadow, float* lit, float* refl, hr_stats* stats);
+
+/* Bare ray queries over device arrays: any_hit != 0 stops at the first hit
+ * (gl_RayFlagsTerminateOnFirstHitEXT); otherwise the closest hit, ties on t
+ * broken by the smaller triangle id. */
+hr_status hr_trace_rays(hr_scene* scene, const hr_ray* rays, uint32_t n, int any_hit,
+                        hr_hit* hits, hr_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HR_HR_H */

@@ -1,0 +1,383 @@
+"""Seeded synthetic scenes for the hybrid pass.
+
+The reference renders hand-made OBJ/glTF scenes (scene.cpp:34-272) whose
+assets are not shipped, so the workloads here are procedural. Every scene is
+returned in the reference's host-side data model:
+  - meshes: (vertices float32[n, 11] = Vertex{position, normal, color, uv}
+    (vk_mesh.h:21-27), indices uint32[m]) (Mesh::_vertices/_indices);
+  - prims: one Primitive per drawable (vk_mesh.h:60-70) with the world matrix
+    entity.m_matrix * node.global (vk_mesh.cpp:476), column-major;
+  - materials: GPUMaterial rows (material.cpp:19-26);
+  - lights: uboLight rows packed as vk_engine.cpp:146-164;
+  - camera: RTCameraData (vk_engine.cpp:730-740) from glm-style lookAt and
+    perspective (camera.cpp:49-58) with projection[1][1] *= -1 (vk_engine.cpp:695).
+
+Workloads (sizes in the names):
+  hybrid_640x360_20k   ground plane + 64 icospheres, 1 point + 1 directional light,
+                       1 shadow ray per light, mirror bounces (depth 8)
+  hybrid_1080p_1m      512x512-quad value-noise terrain + 256 triangle soups of 2048
+                       triangles, 4 rectangular 4x4 area lights, 16 stratified shadow
+                       samples per light, GGX glossy reflections at 4 spp, 1 bounce
+  hybrid_1080p_4m      the same generator with a 1024x1024-quad terrain and 1024 soups
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PRIM_DTYPE = np.dtype([("mesh", "<u4"), ("first_index", "<u4"), ("index_count", "<u4"),
+                       ("material", "<i4"), ("model", "<f4", 16)])
+
+
+@dataclass
+class Scene:
+    name: str
+    meshes: list
+    prims: np.ndarray
+    materials: np.ndarray
+    lights: np.ndarray
+    shapes: object
+    camera: np.ndarray
+    env_sky: np.ndarray
+    env_irr: np.ndarray
+    width: int
+    height: int
+    params: dict = field(default_factory=dict)
+
+    @property
+    def tri_count(self):
+        return int(self.prims["index_count"].sum() // 3)
+
+    @property
+    def prim_tri_offsets(self):
+        c = np.concatenate([[0], np.cumsum(self.prims["index_count"] // 3)])
+        return c.astype(np.int64)
+
+
+# ---------------------------------------------------------------- matrices
+def _colmajor(a):
+    """4x4 row/col math matrix (a[row, col]) -> glm column-major 16 floats."""
+    return np.asarray(a, dtype=np.float64).T.reshape(16)
+
+
+def look_at(eye, center, up=(0.0, 1.0, 0.0)):
+    """glm::lookAt (right-handed), camera.cpp:49-52."""
+    eye, center, up = (np.asarray(v, dtype=np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3] = s
+    m[1, :3] = u
+    m[2, :3] = -f
+    m[0, 3] = -s @ eye
+    m[1, 3] = -u @ eye
+    m[2, 3] = f @ eye
+    return m
+
+
+def perspective(fovy_deg, aspect, near=0.1, far=1000.0):
+    """glm::perspective (depth -1..1), camera.cpp:54-58."""
+    t = np.tan(np.radians(fovy_deg) / 2.0)
+    m = np.zeros((4, 4))
+    m[0, 0] = 1.0 / (aspect * t)
+    m[1, 1] = 1.0 / t
+    m[2, 2] = -(far + near) / (far - near)
+    m[3, 2] = -1.0
+    m[2, 3] = -(2.0 * far * near) / (far - near)
+    return m
+
+
+def rt_camera(eye, center, width, height, fov=60.0, frame=0.0):
+    """RTCameraData as vk_engine.cpp:688-740: inverse(view), inverse(proj) with
+    projection[1][1] *= -1, frame."""
+    view = look_at(eye, center)
+    proj = perspective(fov, width / height)
+    proj[1, 1] *= -1.0
+    cam = np.zeros(36, dtype=np.float32)
+    cam[0:16] = _colmajor(np.linalg.inv(view))
+    cam[16:32] = _colmajor(np.linalg.inv(proj))
+    cam[32:36] = frame
+    return cam
+
+
+def translate_scale(t, s):
+    m = np.eye(4)
+    m[:3, :3] *= s
+    m[:3, 3] = t
+    return m
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+# ---------------------------------------------------------------- materials / lights
+def material(i, albedo, mode, metallic, roughness, ior=1.0):
+    """GPUMaterial as Material::materialToShader (material.cpp:19-26)."""
+    m = np.zeros(12, dtype=np.float32)
+    m[0:3] = albedo
+    m[3] = ior
+    m[4:8] = -1.0
+    m[8:12] = (mode, metallic, roughness, i)
+    return m
+
+
+def point_light(pos, color=(1, 1, 1), intensity=1000.0, max_distance=500.0, radius=1.0):
+    """uboLight for a point light (vk_engine.cpp:157-161); defaults of entity.h:64."""
+    lt = np.zeros(12, dtype=np.float32)
+    lt[0:3] = pos
+    lt[3] = max_distance
+    lt[4:7] = color
+    lt[7] = intensity
+    lt[8] = radius
+    return lt
+
+
+def directional_light(direction, color=(1, 1, 1), intensity=1.0, radius=0.0):
+    """uboLight for a directional light (vk_engine.cpp:152-155): pos.w = -1."""
+    lt = np.zeros(12, dtype=np.float32)
+    lt[0:3] = direction
+    lt[3] = -1.0
+    lt[4:7] = color
+    lt[7] = intensity
+    lt[8] = radius
+    return lt
+
+
+# ---------------------------------------------------------------- environment
+def make_env(sky_w=512, sky_h=256, irr_w=64, irr_h=32):
+    """Analytic sky as equirect float RGBA maps (row 0 = zenith, v = 0.5 - asin(y)/pi)."""
+    def sky_color(w, h, blur):
+        v = (np.arange(h) + 0.5) / h
+        u = (np.arange(w) + 0.5) / w
+        el = (0.5 - v) * np.pi                                  # elevation
+        az = (u - 0.5) * 2 * np.pi
+        el, az = np.meshgrid(el, az, indexing="ij")
+        y = np.sin(el)
+        horizon = np.array([0.80, 0.85, 0.90])
+        zenith = np.array([0.25, 0.45, 0.85])
+        ground = np.array([0.30, 0.27, 0.22])
+        t = np.clip(y, 0, 1)[..., None]
+        col = np.where(y[..., None] >= 0, horizon * (1 - t) + zenith * t,
+                       ground + (horizon - ground) * np.clip(1 + y * 4, 0, 1)[..., None])
+        if not blur:
+            sd = np.array([np.cos(0.6) * np.sin(0.8), np.sin(0.6), np.cos(0.6) * np.cos(0.8)])
+            d = np.stack([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)], -1)
+            sun = np.clip((d @ sd - 0.995) / 0.005, 0, 1)[..., None]
+            col = col * (1 - sun) + np.array([1.0, 0.95, 0.8]) * sun
+        else:
+            col = 0.5 * col + 0.25
+        out = np.ones((h, w, 4), dtype=np.float32)
+        out[..., :3] = col
+        return out
+    return sky_color(sky_w, sky_h, False), sky_color(irr_w, irr_h, True)
+
+
+# ---------------------------------------------------------------- meshes
+def icosphere(subdiv=2):
+    """Unit icosphere: 20 * 4^subdiv triangles, normals = positions."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+             (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    P = np.array(verts)
+    V = np.zeros((len(P), 11), dtype=np.float32)
+    V[:, 0:3] = P
+    V[:, 3:6] = P
+    V[:, 6:9] = 1.0
+    V[:, 9] = 0.5 + np.arctan2(P[:, 0], P[:, 2]) / (2 * np.pi)
+    V[:, 10] = 0.5 - np.arcsin(np.clip(P[:, 1], -1, 1)) / np.pi
+    return V, np.array(faces, dtype=np.uint32).reshape(-1)
+
+
+def quad_mesh(size=100.0):
+    h = size / 2
+    V = np.zeros((4, 11), dtype=np.float32)
+    V[:, 0:3] = [(-h, 0, -h), (h, 0, -h), (h, 0, h), (-h, 0, h)]
+    V[:, 3:6] = (0, 1, 0)
+    V[:, 6:9] = 1
+    V[:, 9:11] = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    return V, np.array([0, 2, 1, 0, 3, 2], dtype=np.uint32)
+
+
+def value_noise(x, z, rng, octaves=6, base_cells=4):
+    """Sum of octaves of smoothstep-interpolated lattice noise in [-1, 1]."""
+    h = np.zeros(np.broadcast(x, z).shape)
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        cells = base_cells * 2 ** o
+        lat = rng.uniform(-1, 1, size=(cells + 2, cells + 2))
+        gx, gz = x * cells, z * cells
+        ix, iz = np.floor(gx).astype(int), np.floor(gz).astype(int)
+        fx, fz = gx - ix, gz - iz
+        sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+        ix, iz = np.clip(ix, 0, cells), np.clip(iz, 0, cells)
+        a = lat[ix, iz] * (1 - sx) + lat[ix + 1, iz] * sx
+        b = lat[ix, iz + 1] * (1 - sx) + lat[ix + 1, iz + 1] * sx
+        h += amp * (a * (1 - sz) + b * sz)
+        total += amp
+        amp *= 0.5
+    return h / total
+
+
+def terrain_mesh(quads, extent, amplitude, rng):
+    n = quads + 1
+    t = np.linspace(0.0, 1.0, n)
+    X, Z = np.meshgrid(t, t, indexing="ij")
+    Hh = amplitude * value_noise(X, Z, rng)
+    px = (X - 0.5) * extent
+    pz = (Z - 0.5) * extent
+    step = extent / quads
+    gx = np.gradient(Hh, step, axis=0)
+    gz = np.gradient(Hh, step, axis=1)
+    nrm = np.stack([-gx, np.ones_like(gx), -gz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    V = np.zeros((n * n, 11), dtype=np.float32)
+    V[:, 0] = px.reshape(-1)
+    V[:, 1] = Hh.reshape(-1)
+    V[:, 2] = pz.reshape(-1)
+    V[:, 3:6] = nrm.reshape(-1, 3)
+    V[:, 6:9] = 1.0
+    V[:, 9] = X.reshape(-1)
+    V[:, 10] = Z.reshape(-1)
+    i, j = np.meshgrid(np.arange(quads), np.arange(quads), indexing="ij")
+    a = (i * n + j).reshape(-1)
+    b = ((i + 1) * n + j).reshape(-1)
+    c = ((i + 1) * n + j + 1).reshape(-1)
+    d = (i * n + j + 1).reshape(-1)
+    idx = np.stack([a, c, b, a, d, c], -1).reshape(-1).astype(np.uint32)
+    height_at = lambda x, z: float(Hh[int(round((x / extent + 0.5) * quads)),
+                                     int(round((z / extent + 0.5) * quads))])
+    return V, idx, height_at
+
+
+def soup_mesh(ntri, rng, tri_size=0.12):
+    """Random triangle soup in the unit cube, face normals, no shared vertices."""
+    c = rng.uniform(-0.5, 0.5, size=(ntri, 1, 3))
+    off = rng.normal(scale=tri_size, size=(ntri, 3, 3))
+    P = (c + off).reshape(-1, 3)
+    e1 = off[:, 1] - off[:, 0]
+    e2 = off[:, 2] - off[:, 0]
+    fn = np.cross(e1, e2)
+    fn /= np.maximum(np.linalg.norm(fn, axis=-1, keepdims=True), 1e-12)
+    V = np.zeros((ntri * 3, 11), dtype=np.float32)
+    V[:, 0:3] = P
+    V[:, 3:6] = np.repeat(fn, 3, axis=0)
+    V[:, 6:9] = 1.0
+    V[:, 9:11] = rng.uniform(0, 1, size=(ntri * 3, 2))
+    return V, np.arange(ntri * 3, dtype=np.uint32)
+
+
+def _prims(rows):
+    p = np.zeros(len(rows), dtype=PRIM_DTYPE)
+    for i, (mesh, first, count, mat, model) in enumerate(rows):
+        p[i] = (mesh, first, count, mat, _colmajor(model).astype(np.float32))
+    return p
+
+
+# ---------------------------------------------------------------- workloads
+def make_small(seed=0, width=640, height=360, spheres=64, frame=0.0):
+    """hybrid_640x360_20k: 2-triangle 100x100 ground + 64 subdivision-2 icospheres
+    (20,482 triangles), centres U([-40,40]x[1,10]x[-40,40]), radii U[0.5,3];
+    albedo U[0,1]^3, metallic 1 (shading model 3) with p = 0.3 else 0 (model 0),
+    roughness 0; one point light at (0,50,0) (radius 0: hard shadows) and one
+    directional light; 1 shadow ray per light, mirror chains up to depth 8."""
+    rng = np.random.default_rng(seed)
+    ground = quad_mesh(100.0)
+    sphere = icosphere(2)
+    mats, rows = [], []
+    n_obj = 1 + spheres
+    for i in range(n_obj):
+        metallic = 1.0 if rng.uniform() < 0.3 else 0.0
+        albedo = rng.uniform(0, 1, size=3)
+        mats.append(material(i, albedo, 3 if metallic else 0, metallic, 0.0))
+    rows.append((0, 0, len(ground[1]), 0, np.eye(4)))
+    for i in range(spheres):
+        c = rng.uniform([-40, 1, -40], [40, 10, 40])
+        r = rng.uniform(0.5, 3.0)
+        rows.append((1, 0, len(sphere[1]), 1 + i, translate_scale(c, r)))
+    lights = np.stack([point_light((0, 50, 0), intensity=1000.0, max_distance=500.0, radius=0.0),
+                       directional_light((0.4, 1.0, 0.3), intensity=1.0)])
+    sky, irr = make_env()
+    return Scene("hybrid_640x360_20k", [ground, sphere], _prims(rows), np.stack(mats), lights, None,
+                 rt_camera((0, 25, 70), (0, 0, 0), width, height, frame=frame), sky, irr, width, height,
+                 dict(shadow_samples=1, refl_spp=1, max_depth=8))
+
+
+def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256, soup_tris=2048,
+                       extent=200.0, samples=16, spp=4, max_depth=1, frame=0.0, name=None):
+    """hybrid_1080p_1m (quads=512, instances=256) / hybrid_1080p_4m (1024, 1024):
+    value-noise terrain (6 octaves, amplitude 20) with a glossy material
+    (shading model 3, metallic 1, roughness U[0,0.6]); triangle-soup instances
+    with uniform positions, uniformly random rotations and scales U[1,5],
+    materials glossy (model 3) with p = 0.5 else PBR (model 0), roughness
+    U[0,0.6]; four 4x4 rectangular area lights at heights U[30,60]."""
+    rng = np.random.default_rng(seed)
+    tV, tI, height_at = terrain_mesh(quads, extent, 20.0, rng)
+    meshes = [(tV, tI)]
+    mats = [material(0, rng.uniform(0.3, 0.9, size=3), 3, 1.0, rng.uniform(0.0, 0.6))]
+    rows = [(0, 0, len(tI), 0, np.eye(4))]
+    half = extent * 0.45
+    for i in range(instances):
+        meshes.append(soup_mesh(soup_tris, rng))
+        glossy = rng.uniform() < 0.5
+        mats.append(material(1 + i, rng.uniform(0, 1, size=3), 3 if glossy else 0,
+                             1.0 if glossy else 0.0, rng.uniform(0.0, 0.6)))
+        s = rng.uniform(1.0, 5.0)
+        x, z = rng.uniform(-half, half, size=2)
+        y = height_at(x, z) + 0.5 * s + 0.5
+        m = translate_scale((x, y, z), s)
+        m[:3, :3] = random_rotation(rng) * s
+        rows.append((1 + i, 0, soup_tris * 3, 1 + i, m))
+    lights, shapes = [], []
+    for _ in range(4):
+        x, z = rng.uniform(-0.3 * extent, 0.3 * extent, size=2)
+        y = rng.uniform(30.0, 60.0)
+        lights.append(point_light((x, y, z), color=rng.uniform(0.7, 1.0, size=3), intensity=3000.0,
+                                  max_distance=500.0, radius=2.0))
+        shapes.append(np.array([1, 4, 0, 0, 0, 0, 4, 0], dtype=np.float32))
+    sky, irr = make_env()
+    cam = rt_camera((0, 0.3 * extent, 0.62 * extent), (0, 0, -0.05 * extent), width, height, frame=frame)
+    nm = name or ("hybrid_%dx%d_%dtri" % (width, height, quads * quads * 2 + instances * soup_tris))
+    return Scene(nm, meshes, _prims(rows), np.stack(mats), np.stack(lights), np.stack(shapes), cam, sky,
+                 irr, width, height, dict(shadow_samples=samples, refl_spp=spp, max_depth=max_depth))
+
+
+def make_1080p_1m(seed=0, **kw):
+    return make_terrain_scene(seed, quads=512, instances=256, extent=200.0, name="hybrid_1080p_1m", **kw)
+
+
+def make_1080p_4m(seed=0, **kw):
+    return make_terrain_scene(seed, quads=1024, instances=1024, extent=400.0, name="hybrid_1080p_4m", **kw)
+
+
+WORKLOADS = {
+    "hybrid_640x360_20k": make_small,
+    "hybrid_1080p_1m": make_1080p_1m,
+    "hybrid_1080p_4m": make_1080p_4m,
+}
