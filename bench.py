@@ -1,0 +1,141 @@
+"""Benchmark of the hybrid pass on one MI355X.
+
+A step is one hybrid pass over a resident G-buffer: the shadow trace
+(shadowRaygen) and the hybrid trace (hybridRaygen + hybridHit: reflection
+bounces and their hit shadow rays). The default workload is the 1920x1080,
+1,048,576-triangle scene (terrain + 256 triangle soups, 4 rectangular area
+lights with 16 stratified shadow samples each, GGX glossy reflections at
+4 spp, 1 bounce). value = traced rays (shadow + reflection + hit shadow) per
+second of the K timed steps; primary rays (the G-buffer) are excluded.
+
+roofline: the shadow kernel (most of the rays) against HBM, with algorithmic
+bytes = nodes visited x 64 B + triangles tested x 48 B + G-buffer position and
+normal read (32 B/px) + shadow factors written (4 B/px/light), from a
+counting launch outside the timed region, over its average launch time
+measured with HIP events on the library's stream.
+
+cpu_baseline: the CPU oracle (test infrastructure; a scalar C++ port of the
+same pass) timed on a row subset of the same frame on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hybrid_1080p_1m")
+    ap.add_argument("--cpu-rows-step", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the hybrid pass is built and measured on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from vulkanhybridraytracer_amd import hr, scenes
+
+    t0 = time.time()
+    sc = scenes.WORKLOADS[args.workload]()
+    log("scene %s: %d triangles, %dx%d, generated in %.1fs" % (sc.name, sc.tri_count, sc.width, sc.height,
+                                                              time.time() - t0))
+    ctx = hr.Context(0)
+    hs = hr.HybridScene(ctx, sc)
+    info = hs.info()
+    log("LBVH build %.2f ms, depth %d" % (info["ms_build"], info["depth"]))
+    rd = hr.HybridRenderer(ctx, hs)
+    gst = rd.rasterize_hybrid(stats=True)  # G-buffer + one pass (also warms up)
+    log("first pass", gst)
+
+    # counting launches (node / triangle visits) outside the timed region
+    flags_saved = rd.params.flags
+    rd.params.flags = 1
+    from vulkanhybridraytracer_amd import abi
+    cs = abi.hr_stats()
+    rd.trace_shadows(cs)
+    shadow_nodes, shadow_tris, shadow_rays = cs.node_visits, cs.tri_tests, cs.shadow_rays
+    rd.params.flags = flags_saved
+
+    for _ in range(args.warmup):
+        rd.trace_shadows()
+        rd.shade_hybrid()
+    ctx.sync()
+
+    ms_shadow, ms_shade = [], []
+    rays = 0
+    st_s, st_h = abi.hr_stats(), abi.hr_stats()
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        rd.trace_shadows(st_s)
+        rd.shade_hybrid(st_h)
+        ms_shadow.append(st_s.ms_shadow)
+        ms_shade.append(st_h.ms_shade)
+        rays += st_s.shadow_rays + st_h.reflection_rays + st_h.hit_shadow_rays
+    ctx.sync()
+    elapsed = time.perf_counter() - t
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = rays / elapsed / 1e9
+    per_step = dict(shadow_rays=int(st_s.shadow_rays), reflection_rays=int(st_h.reflection_rays),
+                    hit_shadow_rays=int(st_h.hit_shadow_rays), occluded_rays=int(st_s.occluded_rays +
+                                                                                  st_h.occluded_rays))
+    W, H, nl = sc.width, sc.height, len(sc.lights)
+    bytes_shadow = shadow_nodes * 64 + shadow_tris * 48 + W * H * (32 + 4 * nl)
+    avg_shadow_ms = float(np.mean(ms_shadow))
+    achieved = bytes_shadow / (avg_shadow_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_shadows", "algorithmic_bytes": int(bytes_shadow),
+                "avg_launch_ms": round(avg_shadow_ms, 3),
+                "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),
+                "tris_per_ray": round(shadow_tris / max(shadow_rays, 1), 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        gb = rd.gb.download()
+        sh = rd.download_shadow()
+        os_ = pyoracle.OracleScene(sc, threads=threads)
+        rows = (0, args.cpu_rows_step)
+        t1 = time.perf_counter()
+        _, c1 = os_.trace_shadows(gb, rows=rows)
+        _, _, c2 = os_.shade_hybrid(gb, sh, rows=rows)
+        tc = time.perf_counter() - t1
+        crays = c1["shadow_rays"] + c2["reflection_rays"] + c2["hit_shadow_rays"]
+        cpu = {"value": round(crays / tc / 1e9, 6), "unit": "Grays/s", "cores": threads, "kind": "port",
+               "sample": "rows 0::%d of the same frame (%d rays, %.1f s)" % (args.cpu_rows_step, crays, tc)}
+        log("cpu baseline", cpu)
+
+    out = {"metric": "Grays/s (1080p, 1M tris)", "value": round(value, 4), "unit": "Grays/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": sc.name, "width": W, "height": H, "triangles": sc.tri_count, "lights": nl,
+                      "shadow_samples": sc.params["shadow_samples"], "refl_spp": sc.params["refl_spp"],
+                      "max_depth": sc.params["max_depth"]},
+           "rays_per_step": per_step, "ms_shadow": round(avg_shadow_ms, 3),
+           "ms_shade": round(float(np.mean(ms_shade)), 3), "ms_gbuffer": round(gst["ms_gbuffer"], 3),
+           "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

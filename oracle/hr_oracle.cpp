@@ -71,6 +71,10 @@ static inline v3 normalize(v3 a) {
     return scl(a, inv);
 }
 static inline float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+/* ordered selection, first operand on ties: box unions do not depend on a
+ * min instruction's sign-of-zero convention */
+static inline float minsel(float a, float b) { return (b < a) ? b : a; }
+static inline float maxsel(float a, float b) { return (b > a) ? b : a; }
 static inline v3 vmin(v3 a, v3 b) { return mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
 static inline v3 vmax(v3 a, v3 b) { return mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
 static inline float comp(v3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
@@ -517,8 +521,8 @@ static void transform_tris(or_scene& s) {
             }
             float* b = &s.tbox[(size_t)g * 6];
             for (int a = 0; a < 3; ++a) {
-                b[a] = fminf(fminf(w[a], w[3 + a]), w[6 + a]);
-                b[3 + a] = fmaxf(fmaxf(w[a], w[3 + a]), w[6 + a]);
+                b[a] = minsel(minsel(w[a], w[3 + a]), w[6 + a]);
+                b[3 + a] = maxsel(maxsel(w[a], w[3 + a]), w[6 + a]);
             }
         }
     }
@@ -601,8 +605,8 @@ static void refit_boxes(or_scene& s) {
         child_box(s, s.kr[i], b);
         float* o = &s.kbox[(size_t)i * 6];
         for (int q = 0; q < 3; ++q) {
-            o[q] = fminf(a[q], b[q]);
-            o[3 + q] = fmaxf(a[3 + q], b[3 + q]);
+            o[q] = minsel(a[q], b[q]);
+            o[3 + q] = maxsel(a[3 + q], b[3 + q]);
         }
     }
 }

@@ -1,0 +1,677 @@
+// hr_api.hip - the extern "C" entry points of include/hr/hr.h.
+//
+// Host side: argument validation (errors are returned with a message, never
+// aborted on), the upload of the scene inputs in the layouts the kernels read,
+// the per-primitive constants (normal matrices), and kernel launches on the
+// context's stream. No computation of the hybrid pass happens on the host.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+using namespace hr;
+
+struct hr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    unsigned long long* d_counters = nullptr;
+    std::string err;
+};
+
+struct hr_scene {
+    hr_context* ctx = nullptr;
+    uint32_t mesh_count = 0, prim_count = 0, ntri = 0, nmat = 0, leaf_max = 4, depth = 0, node_count = 0;
+    float ms_build = 0.0f;
+    std::vector<PrimInfo> h_prims;
+    std::vector<float> h_models;
+    std::vector<uint32_t> h_prim_tri_offset;
+    std::vector<void*> allocs;
+    uint8_t* d_verts = nullptr;
+    float4* d_vnormal = nullptr;
+    uint32_t* d_indices = nullptr;
+    PrimInfo* d_prims = nullptr;
+    float* d_models = nullptr;
+    uint32_t* d_prim_tri_offset = nullptr;
+    hr_material* d_mats = nullptr;
+    float4* d_sky = nullptr;
+    float4* d_irr = nullptr;
+    uint32_t sky_w = 0, sky_h = 0, irr_w = 0, irr_h = 0;
+    BuildBuffers bb{};
+};
+
+namespace {
+
+hr_status fail(hr_context* ctx, hr_status code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+hr_status hip_fail(hr_context* ctx, hipError_t e, const char* what) {
+    hr_status code = (e == hipErrorOutOfMemory) ? HR_E_OOM : HR_E_DEVICE;
+    return fail(ctx, code, "%s: %s", what, hipGetErrorString(e));
+}
+#define HR_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <class T>
+hr_status scene_alloc(hr_scene* s, T** p, size_t count) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 64;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "hipMalloc");
+    s->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return HR_OK;
+}
+#define HR_TRY(x)                     \
+    do {                              \
+        hr_status st_ = (x);          \
+        if (st_ != HR_OK) return st_; \
+    } while (0)
+
+// transpose(inverse(mat3(model))) as cofactors / determinant, row-major
+// (hybridHit.rchit:50); evaluated as written (no contraction)
+void normal_matrix(const float* m, float* nm) {
+    float A00 = m[0], A01 = m[4], A02 = m[8];
+    float A10 = m[1], A11 = m[5], A12 = m[9];
+    float A20 = m[2], A21 = m[6], A22 = m[10];
+    float C00 = A11 * A22 - A12 * A21;
+    float C01 = A12 * A20 - A10 * A22;
+    float C02 = A10 * A21 - A11 * A20;
+    float C10 = A02 * A21 - A01 * A22;
+    float C11 = A00 * A22 - A02 * A20;
+    float C12 = A01 * A20 - A00 * A21;
+    float C20 = A01 * A12 - A02 * A11;
+    float C21 = A02 * A10 - A00 * A12;
+    float C22 = A00 * A11 - A01 * A10;
+    float det = (A00 * C00 + A01 * C01) + A02 * C02;
+    float inv = 1.0f / det;
+    nm[0] = C00 * inv; nm[1] = C01 * inv; nm[2] = C02 * inv;
+    nm[3] = C10 * inv; nm[4] = C11 * inv; nm[5] = C12 * inv;
+    nm[6] = C20 * inv; nm[7] = C21 * inv; nm[8] = C22 * inv;
+}
+
+DevScene dev_scene(const hr_scene* s) {
+    DevScene d;
+    d.nodes = s->bb.nodes;
+    d.tris = s->bb.stris;
+    d.tri_prim = s->bb.tri_prim;
+    d.prims = s->d_prims;
+    d.indices = s->d_indices;
+    d.vnormal = s->d_vnormal;
+    d.mats = s->d_mats;
+    d.sky = s->d_sky;
+    d.irr = s->d_irr;
+    d.node_count = s->node_count;
+    d.tri_count = s->ntri;
+    d.nmat = s->nmat;
+    d.prim_count = s->prim_count;
+    d.sky_w = s->sky_w;
+    d.sky_h = s->sky_h;
+    d.irr_w = s->irr_w;
+    d.irr_h = s->irr_h;
+    return d;
+}
+
+hr_status check_material(hr_context* ctx, const hr_material& m, uint32_t i, uint32_t nmat) {
+    int mode = (int)m.shading_metal_rough_idx[0];
+    if ((float)mode != m.shading_metal_rough_idx[0] || (mode != 0 && mode != 3 && mode != 4))
+        return fail(ctx, HR_E_INVALID, "material %u: shading model %g is not 0, 3 or 4", i,
+                    m.shading_metal_rough_idx[0]);
+    for (int k = 0; k < 4; ++k)
+        if (m.textures[k] > -1.0f)
+            return fail(ctx, HR_E_UNSUPPORTED, "material %u: textured materials are not supported", i);
+    float w = m.shading_metal_rough_idx[3];
+    if (!(w >= 0.0f) || w >= (float)nmat || (float)(int)w != w)
+        return fail(ctx, HR_E_INVALID, "material %u: material index %g out of range", i, w);
+    for (int k = 0; k < 12; ++k) {
+        const float* f = &m.diffuse_ior[0];
+        if (!std::isfinite(f[k])) return fail(ctx, HR_E_INVALID, "material %u: non-finite value", i);
+    }
+    return HR_OK;
+}
+
+hr_status fill_frame(hr_context* ctx, FrameArgs& f, const hr_camera* cam, uint32_t W, uint32_t H) {
+    memset(&f, 0, sizeof(f));
+    if (!cam) return fail(ctx, HR_E_INVALID, "camera is NULL");
+    memcpy(f.inv_view, cam->inv_view, sizeof(f.inv_view));
+    memcpy(f.inv_proj, cam->inv_proj, sizeof(f.inv_proj));
+    f.frame = (uint32_t)(int)cam->frame[0];  // uint(int(cam.frame.x)), hybridRaygen.rgen:29
+    f.width = W;
+    f.height = H;
+    return HR_OK;
+}
+hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, const hr_light_shape* shapes,
+                      uint32_t nl, const hr_trace_params* prm) {
+    if (!lights || nl == 0) return fail(ctx, HR_E_INVALID, "at least one light is required");
+    if (nl > HR_MAX_LIGHTS) return fail(ctx, HR_E_UNSUPPORTED, "%u lights exceed HR_MAX_LIGHTS", nl);
+    if (!prm) return fail(ctx, HR_E_INVALID, "trace params are NULL");
+    if (prm->shadow_samples == 0 || prm->shadow_samples > 4096)
+        return fail(ctx, HR_E_INVALID, "shadow_samples must be in 1..4096");
+    if (prm->refl_spp > 4096 || prm->max_depth > 64)
+        return fail(ctx, HR_E_INVALID, "refl_spp must be <= 4096 and max_depth <= 64");
+    for (uint32_t i = 0; i < nl; ++i) {
+        f.lights[i] = lights[i];
+        const float* v = lights[i].pos_maxdist;
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(v[k])) return fail(ctx, HR_E_INVALID, "light %u: non-finite value", i);
+        if (shapes) {
+            if (shapes[i].type > HR_LIGHT_RECT) return fail(ctx, HR_E_INVALID, "light %u: unknown shape", i);
+            f.shapes[i] = shapes[i];
+        }
+    }
+    f.nl = nl;
+    f.has_shapes = shapes ? 1u : 0u;
+    f.samples = prm->shadow_samples;
+    f.spp = prm->refl_spp;
+    f.max_depth = prm->max_depth;
+    return HR_OK;
+}
+hr_status check_gbuffer(hr_context* ctx, const hr_gbuffer* gb) {
+    if (!gb || !gb->position || !gb->normal || !gb->albedo || !gb->material || !gb->emissive)
+        return fail(ctx, HR_E_INVALID, "G-buffer planes must be non-NULL device pointers");
+    if (gb->width == 0 || gb->height == 0 || gb->width > 16384 || gb->height > 16384)
+        return fail(ctx, HR_E_INVALID, "G-buffer size %ux%u out of range", gb->width, gb->height);
+    return HR_OK;
+}
+GBufView gview(const hr_gbuffer* gb) {
+    return GBufView{reinterpret_cast<float4*>(gb->position), reinterpret_cast<float4*>(gb->normal),
+                    reinterpret_cast<float4*>(gb->albedo), reinterpret_cast<float4*>(gb->material),
+                    reinterpret_cast<float4*>(gb->emissive)};
+}
+
+// Begin / end of a frame call: zero the counters and time with events; with
+// stats the call synchronises and reports counts and milliseconds.
+hr_status frame_begin(hr_context* ctx, bool timing) {
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, launch_zero_u64(ctx->d_counters, C_COUNT, ctx->stream));
+    if (timing) HR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    return HR_OK;
+}
+hr_status frame_end(hr_context* ctx, hr_stats* stats, float* ms_slot) {
+    if (!stats) return HR_OK;
+    HR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HR_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+    unsigned long long c[C_COUNT];
+    HR_HIP(ctx, hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+    float ms = 0.0f;
+    HR_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    float keep[4] = {stats->ms_gbuffer, stats->ms_shadow, stats->ms_shade, stats->ms_build};
+    memset(stats, 0, sizeof(*stats));
+    stats->ms_gbuffer = keep[0];
+    stats->ms_shadow = keep[1];
+    stats->ms_shade = keep[2];
+    stats->ms_build = keep[3];
+    stats->primary_rays = c[C_PRIMARY];
+    stats->shadow_rays = c[C_SHADOW];
+    stats->reflection_rays = c[C_REFL];
+    stats->hit_shadow_rays = c[C_HITSHADOW];
+    stats->occluded_rays = c[C_OCCLUDED];
+    stats->node_visits = c[C_NODES];
+    stats->tri_tests = c[C_TRIS];
+    if (ms_slot) *ms_slot = ms;
+    if (c[C_OVERFLOW]) return fail(ctx, HR_E_OVERFLOW, "traversal stack overflow");
+    return HR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t hr_abi_version(void) { return HR_ABI_VERSION; }
+
+hr_status hr_context_create(int device, hr_context** out) {
+    if (!out) return HR_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return HR_E_DEVICE;
+    if (device < 0 || device >= n) return HR_E_INVALID;
+    hr_context* ctx = new hr_context();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev[1]);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        hr_context_destroy(ctx);
+        return e == hipErrorOutOfMemory ? HR_E_OOM : HR_E_DEVICE;
+    }
+    *out = ctx;
+    return HR_OK;
+}
+
+void hr_context_destroy(hr_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    for (hipEvent_t e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* hr_last_error(const hr_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+hr_status hr_synchronize(hr_context* ctx) {
+    if (!ctx) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HR_OK;
+}
+
+hr_status hr_device_alloc(hr_context* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    HR_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return HR_OK;
+}
+hr_status hr_device_free(hr_context* ctx, void* p) {
+    if (!ctx) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (p) HR_HIP(ctx, hipFree(p));
+    return HR_OK;
+}
+hr_status hr_memcpy_h2d(hr_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (bytes) HR_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return HR_OK;
+}
+hr_status hr_memcpy_d2h(hr_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (bytes) HR_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+hr_status hr_memset_d(hr_context* ctx, void* dst, int value, size_t bytes) {
+    if (!ctx || (bytes && !dst)) return HR_E_INVALID;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    if (bytes) HR_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HR_OK;
+}
+
+void hr_scene_destroy(hr_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    delete s;
+}
+
+hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_count, const hr_primitive* prims,
+                         uint32_t prim_count, const hr_material* materials, uint32_t material_count,
+                         const hr_build_params* params, hr_scene** out) {
+    if (!ctx) return HR_E_INVALID;
+    if (!out) return fail(ctx, HR_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if ((mesh_count && !meshes) || (prim_count && !prims))
+        return fail(ctx, HR_E_INVALID, "meshes/primitives pointer is NULL");
+    if (!materials || material_count == 0) return fail(ctx, HR_E_INVALID, "at least one material is required");
+    uint32_t leaf_max = params && params->leaf_max ? params->leaf_max : 4u;
+    if (leaf_max > LEAF_MAX_LIMIT) return fail(ctx, HR_E_INVALID, "leaf_max %u exceeds %u", leaf_max, LEAF_MAX_LIMIT);
+    for (uint32_t i = 0; i < material_count; ++i) HR_TRY(check_material(ctx, materials[i], i, material_count));
+
+    // concatenated vertices / indices with per-mesh bases
+    std::vector<uint64_t> vbase(mesh_count + 1, 0), ibase(mesh_count + 1, 0);
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        if ((meshes[m].vertex_count && !meshes[m].vertices) || (meshes[m].index_count && !meshes[m].indices))
+            return fail(ctx, HR_E_INVALID, "mesh %u: NULL arrays", m);
+        vbase[m + 1] = vbase[m] + meshes[m].vertex_count;
+        ibase[m + 1] = ibase[m] + meshes[m].index_count;
+        for (uint32_t v = 0; v < meshes[m].vertex_count; ++v) {
+            const float* p = meshes[m].vertices[v].position;
+            if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2]))
+                return fail(ctx, HR_E_INVALID, "mesh %u vertex %u: non-finite position", m, v);
+        }
+    }
+    if (vbase[mesh_count] >= 0xFFFFFFFFull || ibase[mesh_count] >= 0xFFFFFFFFull)
+        return fail(ctx, HR_E_UNSUPPORTED, "too many vertices or indices");
+
+    hr_scene* s = new hr_scene();
+    s->ctx = ctx;
+    s->mesh_count = mesh_count;
+    s->prim_count = prim_count;
+    s->nmat = material_count;
+    s->leaf_max = leaf_max;
+    uint64_t tri = 0;
+    s->h_prims.resize(prim_count);
+    s->h_models.resize((size_t)prim_count * 16);
+    s->h_prim_tri_offset.resize(prim_count + 1);
+    for (uint32_t i = 0; i < prim_count; ++i) {
+        const hr_primitive& hp = prims[i];
+        if (hp.mesh >= mesh_count || hp.index_count % 3 != 0 ||
+            (uint64_t)hp.first_index + hp.index_count > meshes[hp.mesh].index_count || hp.material < 0 ||
+            (uint32_t)hp.material >= material_count) {
+            hr_scene_destroy(s);
+            return fail(ctx, HR_E_INVALID, "primitive %u: mesh, index range or material out of range", i);
+        }
+        const hr_mesh& mh = meshes[hp.mesh];
+        for (uint32_t k = 0; k < hp.index_count; ++k) {
+            if (mh.indices[hp.first_index + k] >= mh.vertex_count) {
+                hr_scene_destroy(s);
+                return fail(ctx, HR_E_INVALID, "primitive %u: vertex index out of range", i);
+            }
+        }
+        for (int k = 0; k < 16; ++k) {
+            if (!std::isfinite(hp.model[k])) {
+                hr_scene_destroy(s);
+                return fail(ctx, HR_E_INVALID, "primitive %u: non-finite model matrix", i);
+            }
+        }
+        PrimInfo& p = s->h_prims[i];
+        memset(&p, 0, sizeof(p));
+        p.tri_offset = (uint32_t)tri;
+        p.index_base = (uint32_t)(ibase[hp.mesh] + hp.first_index);
+        p.vertex_base = (uint32_t)vbase[hp.mesh];
+        p.material = hp.material;
+        normal_matrix(hp.model, p.nm);
+        memcpy(&s->h_models[(size_t)i * 16], hp.model, 16 * sizeof(float));
+        s->h_prim_tri_offset[i] = (uint32_t)tri;
+        tri += hp.index_count / 3;
+    }
+    if (tri >= MAX_TRIS) {
+        hr_scene_destroy(s);
+        return fail(ctx, HR_E_UNSUPPORTED, "%llu triangles exceed the 2^27 limit", (unsigned long long)tri);
+    }
+    s->h_prim_tri_offset[prim_count] = (uint32_t)tri;
+    s->ntri = (uint32_t)tri;
+    uint32_t n = s->ntri;
+    s->node_count = n >= 2 ? n - 1 : (n == 1 ? 1 : 0);
+
+    hr_status st = HR_OK;
+    auto check = [&](hr_status x) {
+        if (st == HR_OK) st = x;
+    };
+    hipError_t he = hipSetDevice(ctx->device);
+    if (he != hipSuccess) {
+        hr_scene_destroy(s);
+        return hip_fail(ctx, he, "hipSetDevice");
+    }
+    uint64_t nv = vbase[mesh_count], ni = ibase[mesh_count];
+    check(scene_alloc(s, &s->d_verts, nv * sizeof(hr_vertex)));
+    check(scene_alloc(s, &s->d_vnormal, nv));
+    check(scene_alloc(s, &s->d_indices, ni));
+    check(scene_alloc(s, &s->d_prims, prim_count));
+    check(scene_alloc(s, &s->d_models, (size_t)prim_count * 16));
+    check(scene_alloc(s, &s->d_prim_tri_offset, prim_count + 1));
+    check(scene_alloc(s, &s->d_mats, material_count));
+    BuildBuffers& b = s->bb;
+    size_t nn = n >= 2 ? n - 1 : 1;
+    check(scene_alloc(s, &b.wtri, (size_t)n * 3));
+    check(scene_alloc(s, &b.tbox, (size_t)n * 2));
+    check(scene_alloc(s, &b.keys, n));
+    check(scene_alloc(s, &b.ids, n));
+    check(scene_alloc(s, &b.keys_tmp, n));
+    check(scene_alloc(s, &b.ids_tmp, n));
+    check(scene_alloc(s, &b.hist, radix_hist_words(n)));
+    check(scene_alloc(s, &b.bounds, 8));
+    check(scene_alloc(s, &b.kl, nn));
+    check(scene_alloc(s, &b.kr, nn));
+    check(scene_alloc(s, &b.kfirst, nn));
+    check(scene_alloc(s, &b.klast, nn));
+    check(scene_alloc(s, &b.kparent, nn));
+    check(scene_alloc(s, &b.lparent, n));
+    check(scene_alloc(s, &b.flags, nn));
+    check(scene_alloc(s, &b.kbox, nn * 2));
+    check(scene_alloc(s, &b.nodes, nn));
+    check(scene_alloc(s, &b.stris, n));
+    check(scene_alloc(s, &b.tri_prim, n));
+    check(scene_alloc(s, &b.depth, 1));
+    if (st != HR_OK) {
+        hr_scene_destroy(s);
+        return st;
+    }
+    // uploads: vertices as-is (44-byte stride), normals as float4 (the
+    // rtVertexAttribute repack of renderer.cpp:3018-3048), indices as-is
+    {
+        std::vector<float4> nrm(nv);
+        std::vector<uint32_t> idx(ni);
+        for (uint32_t m = 0; m < mesh_count; ++m) {
+            if (meshes[m].vertex_count) {
+                he = hipMemcpy(s->d_verts + vbase[m] * sizeof(hr_vertex), meshes[m].vertices,
+                               (size_t)meshes[m].vertex_count * sizeof(hr_vertex), hipMemcpyHostToDevice);
+                if (he != hipSuccess) break;
+            }
+            for (uint32_t v = 0; v < meshes[m].vertex_count; ++v) {
+                const float* q = meshes[m].vertices[v].normal;
+                nrm[vbase[m] + v] = make_float4(q[0], q[1], q[2], 0.0f);
+            }
+            if (meshes[m].index_count)
+                memcpy(&idx[ibase[m]], meshes[m].indices, (size_t)meshes[m].index_count * sizeof(uint32_t));
+        }
+        if (he == hipSuccess && nv)
+            he = hipMemcpy(s->d_vnormal, nrm.data(), nv * sizeof(float4), hipMemcpyHostToDevice);
+        if (he == hipSuccess && ni)
+            he = hipMemcpy(s->d_indices, idx.data(), ni * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+    if (he == hipSuccess && prim_count)
+        he = hipMemcpy(s->d_prims, s->h_prims.data(), prim_count * sizeof(PrimInfo), hipMemcpyHostToDevice);
+    if (he == hipSuccess && prim_count)
+        he = hipMemcpy(s->d_models, s->h_models.data(), s->h_models.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+        he = hipMemcpy(s->d_prim_tri_offset, s->h_prim_tri_offset.data(), (prim_count + 1) * sizeof(uint32_t),
+                       hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+        he = hipMemcpy(s->d_mats, materials, material_count * sizeof(hr_material), hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        hr_scene_destroy(s);
+        return hip_fail(ctx, he, "scene upload");
+    }
+    b.verts = s->d_verts;
+    b.indices = s->d_indices;
+    b.prims = s->d_prims;
+    b.models = s->d_models;
+    b.prim_tri_offset = s->d_prim_tri_offset;
+    b.prim_count = prim_count;
+    b.tri_count = n;
+    b.leaf_max = leaf_max;
+
+    he = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (he == hipSuccess) he = lbvh_build(b, ctx->stream);
+    if (he == hipSuccess) he = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(ctx->ev[1]);
+    if (he == hipSuccess) he = hipEventElapsedTime(&s->ms_build, ctx->ev[0], ctx->ev[1]);
+    uint32_t depth = 0;
+    if (he == hipSuccess && n > 0) he = hipMemcpy(&depth, b.depth, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) {
+        hr_scene_destroy(s);
+        return hip_fail(ctx, he, "LBVH build");
+    }
+    s->depth = depth;
+    if (depth > MAX_DEPTH) {
+        hr_scene_destroy(s);
+        return fail(ctx, HR_E_OVERFLOW, "BVH depth %u exceeds the traversal stack (%u)", depth, MAX_DEPTH);
+    }
+    *out = s;
+    return HR_OK;
+}
+
+hr_status hr_scene_refit(hr_scene* s, const float* models) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (!models) return fail(ctx, HR_E_INVALID, "models is NULL");
+    for (uint32_t i = 0; i < s->prim_count; ++i) {
+        for (int k = 0; k < 16; ++k)
+            if (!std::isfinite(models[16 * i + k]))
+                return fail(ctx, HR_E_INVALID, "primitive %u: non-finite model matrix", i);
+    }
+    for (uint32_t i = 0; i < s->prim_count; ++i) {
+        memcpy(&s->h_models[(size_t)i * 16], models + 16 * (size_t)i, 16 * sizeof(float));
+        normal_matrix(models + 16 * (size_t)i, s->h_prims[i].nm);
+    }
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (s->prim_count) {
+        HR_HIP(ctx, hipMemcpy(s->d_models, s->h_models.data(), s->h_models.size() * sizeof(float),
+                              hipMemcpyHostToDevice));
+        HR_HIP(ctx, hipMemcpy(s->d_prims, s->h_prims.data(), s->prim_count * sizeof(PrimInfo),
+                              hipMemcpyHostToDevice));
+    }
+    HR_HIP(ctx, lbvh_refit(s->bb, ctx->stream));
+    uint32_t depth = 0;
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (s->ntri) HR_HIP(ctx, hipMemcpy(&depth, s->bb.depth, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    s->depth = depth;
+    return HR_OK;
+}
+
+hr_status hr_scene_set_materials(hr_scene* s, const hr_material* materials, uint32_t count) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (!materials || count != s->nmat)
+        return fail(ctx, HR_E_INVALID, "expected %u materials", s->nmat);
+    for (uint32_t i = 0; i < count; ++i) HR_TRY(check_material(ctx, materials[i], i, count));
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HR_HIP(ctx, hipMemcpy(s->d_mats, materials, count * sizeof(hr_material), hipMemcpyHostToDevice));
+    return HR_OK;
+}
+
+hr_status hr_scene_set_env(hr_scene* s, const hr_env* env) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (!env || !env->skybox || !env->irradiance || !env->sky_width || !env->sky_height || !env->irr_width ||
+        !env->irr_height)
+        return fail(ctx, HR_E_INVALID, "environment maps must be non-empty");
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    size_t ns = (size_t)env->sky_width * env->sky_height, ni = (size_t)env->irr_width * env->irr_height;
+    if (ns != (size_t)s->sky_w * s->sky_h) {
+        HR_TRY(scene_alloc(s, &s->d_sky, ns));
+    }
+    if (ni != (size_t)s->irr_w * s->irr_h) {
+        HR_TRY(scene_alloc(s, &s->d_irr, ni));
+    }
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HR_HIP(ctx, hipMemcpy(s->d_sky, env->skybox, ns * sizeof(float4), hipMemcpyHostToDevice));
+    HR_HIP(ctx, hipMemcpy(s->d_irr, env->irradiance, ni * sizeof(float4), hipMemcpyHostToDevice));
+    s->sky_w = env->sky_width;
+    s->sky_h = env->sky_height;
+    s->irr_w = env->irr_width;
+    s->irr_h = env->irr_height;
+    return HR_OK;
+}
+
+hr_status hr_scene_info(const hr_scene* s, uint32_t* tri_count, uint32_t* node_count, uint32_t* depth,
+                        float* ms_build) {
+    if (!s) return HR_E_INVALID;
+    if (tri_count) *tri_count = s->ntri;
+    if (node_count) *node_count = s->node_count;
+    if (depth) *depth = s->depth;
+    if (ms_build) *ms_build = s->ms_build;
+    return HR_OK;
+}
+
+hr_status hr_scene_export(const hr_scene* s, uint32_t which, void* dst, size_t bytes) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    const void* src = nullptr;
+    size_t n = 0;
+    uint32_t nt = s->ntri;
+    std::vector<uint32_t> tmp;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    switch (which) {
+        case 0: src = s->bb.nodes; n = (size_t)s->node_count * sizeof(TNode); break;
+        case 1: src = s->bb.stris; n = (size_t)nt * sizeof(STri); break;
+        case 2: src = s->bb.keys; n = (size_t)nt * 4; break;
+        case 3: src = s->bb.ids; n = (size_t)nt * 4; break;
+        case 4: {
+            if (nt >= 2) {
+                size_t m = nt - 1;
+                std::vector<uint32_t> a(m), b(m), c(m), d(m);
+                HR_HIP(ctx, hipMemcpy(a.data(), s->bb.kl, m * 4, hipMemcpyDeviceToHost));
+                HR_HIP(ctx, hipMemcpy(b.data(), s->bb.kr, m * 4, hipMemcpyDeviceToHost));
+                HR_HIP(ctx, hipMemcpy(c.data(), s->bb.kfirst, m * 4, hipMemcpyDeviceToHost));
+                HR_HIP(ctx, hipMemcpy(d.data(), s->bb.klast, m * 4, hipMemcpyDeviceToHost));
+                tmp.resize(m * 4);
+                for (size_t i = 0; i < m; ++i) {
+                    tmp[4 * i] = a[i]; tmp[4 * i + 1] = b[i]; tmp[4 * i + 2] = c[i]; tmp[4 * i + 3] = d[i];
+                }
+            }
+            if (bytes < tmp.size() * 4) return fail(ctx, HR_E_INVALID, "export buffer too small");
+            if (!tmp.empty()) memcpy(dst, tmp.data(), tmp.size() * 4);
+            return HR_OK;
+        }
+        default: return fail(ctx, HR_E_INVALID, "unknown export %u", which);
+    }
+    if (bytes < n) return fail(ctx, HR_E_INVALID, "export buffer too small");
+    if (n) HR_HIP(ctx, hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+    return HR_OK;
+}
+
+hr_status hr_gbuffer_primary(hr_scene* s, const hr_camera* cam, const hr_gbuffer* gb, hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    HR_TRY(check_gbuffer(ctx, gb));
+    FrameArgs f;
+    HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_gbuffer(dev_scene(s), f, gview(gb), ctx->d_counters, false, ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_gbuffer : nullptr);
+}
+
+hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* lights, const hr_light_shape* shapes,
+                           uint32_t nl, const hr_camera* cam, const hr_trace_params* params, float* shadow,
+                           hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    HR_TRY(check_gbuffer(ctx, gb));
+    if (!shadow) return fail(ctx, HR_E_INVALID, "shadow output is NULL");
+    FrameArgs f;
+    HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
+    HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
+    bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_shadows(dev_scene(s), f, gview(gb), shadow, ctx->d_counters, count, ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
+}
+
+hr_status hr_shade_hybrid(hr_scene* s, const hr_gbuffer* gb, const hr_light* lights, const hr_light_shape* shapes,
+                          uint32_t nl, const hr_camera* cam, const hr_trace_params* params, const float* shadow,
+                          float* lit, float* refl, hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    HR_TRY(check_gbuffer(ctx, gb));
+    if (!shadow || !lit || !refl) return fail(ctx, HR_E_INVALID, "shadow/lit/refl pointer is NULL");
+    FrameArgs f;
+    HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
+    HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
+    bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_shade(dev_scene(s), f, gview(gb), shadow, reinterpret_cast<float4*>(lit),
+                             reinterpret_cast<float4*>(refl), ctx->d_counters, count, ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
+}
+
+hr_status hr_trace_rays(hr_scene* s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits, hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (n && (!rays || !hits)) return fail(ctx, HR_E_INVALID, "rays/hits pointer is NULL");
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_trace_rays(dev_scene(s), rays, n, any_hit, hits, ctx->d_counters, stats != nullptr,
+                                  ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// internal.h - data layouts in HBM and launcher interfaces of libhr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "hr/hr.h"
+
+namespace hr {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t MAX_DEPTH = 64;       // traversal stack bound (checked at build)
+constexpr uint32_t STACK_LDS = 16;       // entries per lane kept in LDS
+constexpr uint32_t STACK_SPILL = MAX_DEPTH - STACK_LDS;  // entries per lane in private memory
+constexpr uint32_t LEAF_MAX_LIMIT = 16;  // leaf code holds count - 1 in 4 bits
+constexpr uint32_t MAX_TRIS = 1u << 27;  // leaf code holds the first triangle in 27 bits
+
+__host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) { return LEAF_BIT | ((count - 1u) << 27) | first; }
+
+// Traversal node, 64 B, one per Karras internal node:
+//   a = (child 0 box min, child 0 code)   b = (child 0 box max, child 1 code)
+//   c = (child 1 box min, 0)              d = (child 1 box max, 0)
+// A code below LEAF_BIT is an internal node index; a leaf code is
+// LEAF_BIT | (count - 1) << 27 | first sorted triangle.
+struct __align__(64) TNode {
+    float4 a, b, c, d;
+};
+// Sorted world-space triangle, 48 B: (v0, triangle id), (v1, 0), (v2, 0).
+struct __align__(16) STri {
+    float4 v0, v1, v2;
+};
+// Per primitive constants for shading, 64 B.
+struct __align__(16) PrimInfo {
+    uint32_t tri_offset;   // first global triangle id of the primitive
+    uint32_t index_base;   // offset of its first index in the concatenated index array
+    uint32_t vertex_base;  // offset of its mesh's first vertex in the concatenated vertex arrays
+    int32_t material;
+    float nm[9];           // transpose(inverse(mat3(model))), row-major
+    uint32_t _pad[3];
+};
+static_assert(sizeof(TNode) == 64, "TNode");
+static_assert(sizeof(STri) == 48, "STri");
+static_assert(sizeof(PrimInfo) == 64, "PrimInfo");
+
+// Read-only view of a built scene, passed to kernels by value.
+struct DevScene {
+    const TNode* nodes;
+    const STri* tris;
+    const uint32_t* tri_prim;   // global triangle id -> primitive
+    const PrimInfo* prims;
+    const uint32_t* indices;    // concatenated mesh indices
+    const float4* vnormal;      // concatenated per-vertex normals (xyz, 0)
+    const hr_material* mats;
+    const float4* sky;
+    const float4* irr;
+    uint32_t node_count, tri_count, nmat, prim_count;
+    uint32_t sky_w, sky_h, irr_w, irr_h;
+};
+
+// Per-frame uniforms, passed to kernels by value (replaces the camera, light,
+// samples UBO/SSBOs of renderer.cpp:1943-2028 and 2946-3123).
+struct FrameArgs {
+    hr_light lights[HR_MAX_LIGHTS];
+    hr_light_shape shapes[HR_MAX_LIGHTS];
+    float inv_view[16];
+    float inv_proj[16];
+    uint32_t nl;
+    uint32_t frame;
+    uint32_t width, height;
+    uint32_t samples, spp, max_depth;
+    uint32_t has_shapes;
+};
+
+struct GBufView {
+    float4* pos;
+    float4* nrm;
+    float4* alb;
+    float4* mat;
+    float4* emi;
+};
+
+// counters slots (uint64)
+enum : int {
+    C_PRIMARY = 0,
+    C_SHADOW = 1,
+    C_REFL = 2,
+    C_HITSHADOW = 3,
+    C_OCCLUDED = 4,
+    C_NODES = 5,
+    C_TRIS = 6,
+    C_OVERFLOW = 7,
+    C_COUNT = 8
+};
+
+// ---- launchers (trace.hip) ---------------------------------------------------
+hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, unsigned long long* counters,
+                          bool count, hipStream_t st);
+hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
+                          unsigned long long* counters, bool count, hipStream_t st);
+hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, const float* shadow, float4* lit,
+                        float4* refl, unsigned long long* counters, bool count, hipStream_t st);
+hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits,
+                             unsigned long long* counters, bool count, hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+
+// ---- LBVH build (lbvh.hip) ------------------------------------------------------
+struct BuildBuffers {
+    // inputs
+    const uint8_t* verts;       // concatenated 44-byte vertices
+    const uint32_t* indices;    // concatenated indices
+    const PrimInfo* prims;
+    const float* models;        // prim_count x 16
+    const uint32_t* prim_tri_offset;  // prim_count + 1
+    uint32_t prim_count, tri_count, leaf_max;
+    // outputs / scratch
+    float4* wtri;       // tri_count x 3 (primitive order)
+    float4* tbox;       // tri_count x 2
+    uint32_t* keys;     // sorted Morton codes
+    uint32_t* ids;      // sorted triangle ids
+    uint32_t* keys_tmp;
+    uint32_t* ids_tmp;
+    uint32_t* hist;     // radix histogram, 256 x blocks
+    uint32_t* bounds;   // 6 ordered-uint centroid bounds
+    uint32_t* kl;       // Karras children
+    uint32_t* kr;
+    uint32_t* kfirst;
+    uint32_t* klast;
+    uint32_t* kparent;  // internal node -> parent
+    uint32_t* lparent;  // sorted leaf -> parent
+    uint32_t* flags;    // refit arrival counters
+    float4* kbox;       // internal node boxes x 2
+    TNode* nodes;
+    STri* stris;
+    uint32_t* tri_prim;
+    uint32_t* depth;    // 1 uint
+};
+size_t radix_hist_words(uint32_t n);
+hipError_t lbvh_build(BuildBuffers& b, hipStream_t st);   // full build
+hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st);   // new models, same topology
+
+}  // namespace hr

@@ -1,0 +1,444 @@
+// lbvh.hip - on-GPU linear BVH over world-space triangles (replaces the
+// driver BLAS/TLAS build of renderer.cpp:1643-1916).
+//
+// 1. k_transform   world-space triangles (model * position, vk_mesh.cpp:476;
+//                  44-byte vertices, uint32 indices, vk_mesh.cpp:436-442),
+//                  triangle boxes and the centroid bounds (ordered-uint atomics)
+// 2. k_morton      30-bit Morton code of each box centre on a 1024^3 grid
+// 3. radix sort    stable LSD sort of (code, triangle id), four 8-bit digits:
+//                  per-tile digit histograms, one exclusive scan, and a scatter
+//                  whose in-tile ranks come from 64-lane ballot matching, so
+//                  equal codes keep triangle-id order
+// 4. k_karras      Karras (HPG 2012) hierarchy from the sorted keys, ties
+//                  broken by the sorted index
+// 5. k_refit       bottom-up boxes, the second child to arrive builds the
+//                  parent (acquire/release arrival counters)
+// 6. k_layout      64-byte two-child-box nodes, subtrees of <= leaf_max
+//                  triangles collapsed into leaf ranges; k_gather writes the
+//                  sorted 48-byte triangles
+// Every step is integer work or exact min/max, so the node and triangle arrays
+// are bit-identical to a CPU build of the same definitions.
+#include "device_math.h"
+#include "internal.h"
+
+namespace hr {
+using namespace hrd;
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int SCAN_THREADS = 1024;
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_init_bounds(uint32_t* b) {
+    if (threadIdx.x < 3) b[threadIdx.x] = 0xFFFFFFFFu;
+    else if (threadIdx.x < 6) b[threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ uint32_t find_prim(const uint32_t* off, uint32_t np, uint32_t g) {
+    uint32_t lo = 0, hi = np;  // largest p with off[p] <= g
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_transform(BuildBuffers b, int with_bounds) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = g < b.tri_count;
+    uint32_t omin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t omax[3] = {0u, 0u, 0u};
+    if (valid) {
+        uint32_t p = find_prim(b.prim_tri_offset, b.prim_count, g);
+        const PrimInfo pi = b.prims[p];
+        const float* m = b.models + 16 * p;
+        float M[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) M[i] = m[i];
+        uint32_t local = g - pi.tri_offset;
+        float w[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint32_t vi = b.indices[pi.index_base + 3 * local + k];
+            const float* pos = (const float*)(b.verts + (size_t)(pi.vertex_base + vi) * 44u);
+            float o[4];
+            mul4(M, pos[0], pos[1], pos[2], 1.0f, o);
+            w[3 * k + 0] = o[0];
+            w[3 * k + 1] = o[1];
+            w[3 * k + 2] = o[2];
+            b.wtri[3 * (size_t)g + k] = make_float4(o[0], o[1], o[2], 0.0f);
+        }
+        float lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = minsel(minsel(w[a], w[3 + a]), w[6 + a]);
+            hi[a] = maxsel(maxsel(w[a], w[3 + a]), w[6 + a]);
+        }
+        b.tbox[2 * (size_t)g + 0] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        b.tbox[2 * (size_t)g + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+        b.tri_prim[g] = p;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t oc = f2ord((lo[a] + hi[a]) * 0.5f);
+            omin[a] = oc;
+            omax[a] = oc;
+        }
+    }
+    if (!with_bounds) return;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            omin[a] = min(omin[a], (uint32_t)__shfl_xor((int)omin[a], off, 64));
+            omax[a] = max(omax[a], (uint32_t)__shfl_xor((int)omax[a], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&b.bounds[a], omin[a]);
+            atomicMax(&b.bounds[3 + a], omax[a]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_morton(BuildBuffers b) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.tri_count) return;
+    float cmin[3], cmax[3], scale[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cmin[a] = ord2f(b.bounds[a]);
+        cmax[a] = ord2f(b.bounds[3 + a]);
+        float ext = cmax[a] - cmin[a];
+        scale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+    }
+    float4 lo = b.tbox[2 * (size_t)g], hi = b.tbox[2 * (size_t)g + 1];
+    float cx = (lo.x + hi.x) * 0.5f, cy = (lo.y + hi.y) * 0.5f, cz = (lo.z + hi.z) * 0.5f;
+    b.keys[g] = morton3(quant10(cx, cmin[0], scale[0]), quant10(cy, cmin[1], scale[1]),
+                        quant10(cz, cmin[2], scale[2]));
+    b.ids[g] = g;
+}
+
+// ---- radix sort -------------------------------------------------------------
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, uint32_t n, int shift,
+                                                        uint32_t* hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint32_t idx = base + j * RS_THREADS + tid;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)tid * nblocks + blockIdx.x] = h[tid];
+}
+
+// exclusive scan of len words in one workgroup
+__global__ __launch_bounds__(SCAN_THREADS) void k_rs_scan(uint32_t* a, uint32_t len) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    __shared__ uint32_t carry_s;
+    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < len; base += SCAN_THREADS * 4) {
+        uint32_t v[4];
+        uint32_t i0 = base + tid * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (i0 + k < len) ? a[i0 + k] : 0u;
+        uint32_t s = v[0] + v[1] + v[2] + v[3];
+        uint32_t inc = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t t = (uint32_t)__shfl_up((int)inc, off, 64);
+            if (lane >= off) inc += t;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t x = lane < SCAN_THREADS / 64 ? wsum[lane] : 0u;
+            uint32_t y = x;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                uint32_t t = (uint32_t)__shfl_up((int)y, off, 64);
+                if (lane >= off) y += t;
+            }
+            if (lane < SCAN_THREADS / 64) wsum[lane] = y - x;  // exclusive wave offsets
+        }
+        __syncthreads();
+        uint32_t run = carry_s + wsum[w] + inc - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < len) a[i0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == SCAN_THREADS - 1) carry_s = run;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                           uint32_t* vout, uint32_t n, int shift,
+                                                           const uint32_t* hist, uint32_t nblocks) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    base[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < RS_THREADS / 64; ++q) wcnt[q][tid] = 0;
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        uint32_t idx = blockIdx.x * RS_TILE + j * RS_THREADS + tid;
+        bool valid = idx < n;
+        uint32_t key = valid ? kin[idx] : 0u;
+        uint32_t val = valid ? vin[idx] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            bool on = (d >> bit) & 1u;
+            unsigned long long bb = __ballot(on);
+            m &= on ? bb : ~bb;
+        }
+        uint32_t r = (uint32_t)__popcll(m & lt);
+        if (valid && r == 0) wcnt[w][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + r;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < RS_THREADS / 64; ++q) {
+            tot += wcnt[q][tid];
+            wcnt[q][tid] = 0;
+        }
+        base[tid] += tot;
+        __syncthreads();
+    }
+}
+
+// ---- Karras hierarchy -------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint32_t* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ __launch_bounds__(256) void k_karras(BuildBuffers b) {
+    int n = (int)b.tri_count;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint32_t* k = b.keys;
+    int d = (kdelta(k, n, i, i + 1) - kdelta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = kdelta(k, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (kdelta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = kdelta(k, n, i, j);
+    int sp = 0;
+    int div = 2;
+    for (;;) {
+        int t = (l + div - 1) / div;
+        if (kdelta(k, n, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t <= 1) break;
+        div *= 2;
+    }
+    int gamma = i + sp * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    uint32_t left = (lo == gamma) ? (LEAF_BIT | (uint32_t)gamma) : (uint32_t)gamma;
+    uint32_t right = (hi == gamma + 1) ? (LEAF_BIT | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+    b.kl[i] = left;
+    b.kr[i] = right;
+    b.kfirst[i] = (uint32_t)lo;
+    b.klast[i] = (uint32_t)hi;
+    if (left & LEAF_BIT) b.lparent[left & ~LEAF_BIT] = (uint32_t)i; else b.kparent[left] = (uint32_t)i;
+    if (right & LEAF_BIT) b.lparent[right & ~LEAF_BIT] = (uint32_t)i; else b.kparent[right] = (uint32_t)i;
+    if (i == 0) b.kparent[0] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void child_box(const BuildBuffers& b, uint32_t c, float4& lo, float4& hi) {
+    if (c & LEAF_BIT) {
+        uint32_t g = b.ids[c & ~LEAF_BIT];
+        lo = b.tbox[2 * (size_t)g];
+        hi = b.tbox[2 * (size_t)g + 1];
+    } else {
+        lo = b.kbox[2 * (size_t)c];
+        hi = b.kbox[2 * (size_t)c + 1];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_refit(BuildBuffers b) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.tri_count) return;
+    uint32_t p = b.lparent[k];
+    for (;;) {
+        uint32_t old = __hip_atomic_fetch_add(&b.flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;
+        float4 alo, ahi, blo, bhi;
+        child_box(b, b.kl[p], alo, ahi);
+        child_box(b, b.kr[p], blo, bhi);
+        b.kbox[2 * (size_t)p] = make_float4(minsel(alo.x, blo.x), minsel(alo.y, blo.y), minsel(alo.z, blo.z), 0.0f);
+        b.kbox[2 * (size_t)p + 1] =
+            make_float4(maxsel(ahi.x, bhi.x), maxsel(ahi.y, bhi.y), maxsel(ahi.z, bhi.z), 0.0f);
+        if (p == 0) return;
+        p = b.kparent[p];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_layout(BuildBuffers b) {
+    uint32_t n = b.tri_count;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 1) {
+        if (i == 0) {
+            uint32_t g = b.ids[0];
+            float4 lo = b.tbox[2 * (size_t)g], hi = b.tbox[2 * (size_t)g + 1];
+            TNode nd;
+            nd.a = make_float4(lo.x, lo.y, lo.z, __uint_as_float(leaf_code(0, 1)));
+            nd.b = make_float4(hi.x, hi.y, hi.z, __uint_as_float(leaf_code(0, 1)));
+            nd.c = make_float4(lo.x, lo.y, lo.z, 0.0f);
+            nd.d = make_float4(hi.x, hi.y, hi.z, 0.0f);
+            b.nodes[0] = nd;
+        }
+        return;
+    }
+    if (i >= n - 1) return;
+    uint32_t ch[2] = {b.kl[i], b.kr[i]};
+    uint32_t code[2];
+    float4 lo[2], hi[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        child_box(b, ch[c], lo[c], hi[c]);
+        if (ch[c] & LEAF_BIT) {
+            code[c] = leaf_code(ch[c] & ~LEAF_BIT, 1);
+        } else {
+            uint32_t sz = b.klast[ch[c]] - b.kfirst[ch[c]] + 1;
+            code[c] = sz <= b.leaf_max ? leaf_code(b.kfirst[ch[c]], sz) : ch[c];
+        }
+    }
+    TNode nd;
+    nd.a = make_float4(lo[0].x, lo[0].y, lo[0].z, __uint_as_float(code[0]));
+    nd.b = make_float4(hi[0].x, hi[0].y, hi[0].z, __uint_as_float(code[1]));
+    nd.c = make_float4(lo[1].x, lo[1].y, lo[1].z, 0.0f);
+    nd.d = make_float4(hi[1].x, hi[1].y, hi[1].z, 0.0f);
+    b.nodes[i] = nd;
+}
+
+__global__ __launch_bounds__(256) void k_gather(BuildBuffers b) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.tri_count) return;
+    uint32_t g = b.ids[k];
+    float4 v0 = b.wtri[3 * (size_t)g], v1 = b.wtri[3 * (size_t)g + 1], v2 = b.wtri[3 * (size_t)g + 2];
+    STri t;
+    t.v0 = make_float4(v0.x, v0.y, v0.z, __uint_as_float(g));
+    t.v1 = make_float4(v1.x, v1.y, v1.z, 0.0f);
+    t.v2 = make_float4(v2.x, v2.y, v2.z, 0.0f);
+    b.stris[k] = t;
+}
+
+// deepest leaf path counted in traversal nodes (the root and subtrees with
+// more than leaf_max triangles)
+__global__ __launch_bounds__(256) void k_depth(BuildBuffers b) {
+    uint32_t n = b.tri_count;
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t d = 0;
+    if (n >= 2) {
+        uint32_t a = b.lparent[k];
+        for (;;) {
+            uint32_t sz = b.klast[a] - b.kfirst[a] + 1;
+            if (a == 0 || sz > b.leaf_max) ++d;
+            if (a == 0) break;
+            a = b.kparent[a];
+        }
+    } else {
+        d = 1;
+    }
+    atomicMax(b.depth, d);
+}
+
+inline uint32_t blocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+
+hipError_t radix_sort(BuildBuffers& b, hipStream_t st) {
+    uint32_t n = b.tri_count;
+    uint32_t nb = blocks(n, RS_TILE);
+    uint32_t* kin = b.keys;
+    uint32_t* vin = b.ids;
+    uint32_t* kout = b.keys_tmp;
+    uint32_t* vout = b.ids_tmp;
+    for (int pass = 0; pass < 4; ++pass) {
+        int shift = pass * 8;
+        k_rs_hist<<<nb, RS_THREADS, 0, st>>>(kin, n, shift, b.hist, nb);
+        k_rs_scan<<<1, SCAN_THREADS, 0, st>>>(b.hist, 256u * nb);
+        k_rs_scatter<<<nb, RS_THREADS, 0, st>>>(kin, vin, kout, vout, n, shift, b.hist, nb);
+        uint32_t* t = kin; kin = kout; kout = t;
+        t = vin; vin = vout; vout = t;
+    }
+    return hipGetLastError();  // four passes end in keys/ids
+}
+
+hipError_t finish(BuildBuffers& b, hipStream_t st) {
+    uint32_t n = b.tri_count;
+    if (n >= 2) {
+        k_fill_u32<<<blocks(n - 1, 256), 256, 0, st>>>(b.flags, n - 1, 0u);
+        k_refit<<<blocks(n, 256), 256, 0, st>>>(b);
+    }
+    uint32_t nn = n >= 2 ? n - 1 : 1;
+    k_layout<<<blocks(nn, 256), 256, 0, st>>>(b);
+    k_gather<<<blocks(n, 256), 256, 0, st>>>(b);
+    k_fill_u32<<<1, 64, 0, st>>>(b.depth, 1, 0u);
+    k_depth<<<blocks(n, 256), 256, 0, st>>>(b);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t radix_hist_words(uint32_t n) { return 256u * (size_t)blocks(n > 0 ? n : 1, RS_TILE); }
+
+hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
+    uint32_t n = b.tri_count;
+    if (n == 0) return hipSuccess;
+    k_init_bounds<<<1, 64, 0, st>>>(b.bounds);
+    k_transform<<<blocks(n, 256), 256, 0, st>>>(b, 1);
+    k_morton<<<blocks(n, 256), 256, 0, st>>>(b);
+    hipError_t e = radix_sort(b, st);
+    if (e != hipSuccess) return e;
+    if (n >= 2) k_karras<<<blocks(n - 1, 256), 256, 0, st>>>(b);
+    return finish(b, st);
+}
+
+hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st) {
+    uint32_t n = b.tri_count;
+    if (n == 0) return hipSuccess;
+    k_transform<<<blocks(n, 256), 256, 0, st>>>(b, 0);
+    return finish(b, st);
+}
+
+}  // namespace hr

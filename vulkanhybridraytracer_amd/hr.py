@@ -1,0 +1,308 @@
+"""Python host binding of libhr.so (the C ABI of include/hr/hr.h).
+
+The classes mirror the reference's host objects for the hybrid pass:
+  Context         the device + queue the reference gets from vk-bootstrap
+  HybridScene     Scene + its acceleration structure (create_bottom/top_
+                  acceleration_structure, renderer.cpp:1643-1688), refit as
+                  buildTlas(update=true) (vk_engine.cpp:174-187)
+  GBuffer         the six offscreen attachments of renderer.cpp:252-269 (FP32)
+  HybridRenderer  Renderer::rasterize_hybrid (renderer.cpp:530-600): G-buffer,
+                  shadow trace, hybrid trace, each on the GPU
+
+The library must be built (python -m vulkanhybridraytracer_amd.build); there
+is no fallback: every call goes to the HIP kernels.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhr.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "hr", "hr.h")
+
+
+class HRError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def exported_functions():
+    """Names of the functions declared in include/hr/hr.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:hr_status|void|uint32_t|const char\*)\s+(hr_\w+)\s*\(", text, re.M)))
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libhr.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libhr.so is not built: run python -m vulkanhybridraytracer_amd.build")
+    L = C.CDLL(path)
+    vp, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
+    st = C.c_int
+    sigs = {
+        "hr_abi_version": (u32, []),
+        "hr_context_create": (st, [i32, C.POINTER(vp)]),
+        "hr_context_destroy": (None, [vp]),
+        "hr_last_error": (C.c_char_p, [vp]),
+        "hr_synchronize": (st, [vp]),
+        "hr_device_alloc": (st, [vp, sz, C.POINTER(vp)]),
+        "hr_device_free": (st, [vp, vp]),
+        "hr_memcpy_h2d": (st, [vp, vp, vp, sz]),
+        "hr_memcpy_d2h": (st, [vp, vp, vp, sz]),
+        "hr_memset_d": (st, [vp, vp, i32, sz]),
+        "hr_scene_build": (st, [vp, vp, u32, vp, u32, vp, u32, C.POINTER(abi.hr_build_params), C.POINTER(vp)]),
+        "hr_scene_refit": (st, [vp, vp]),
+        "hr_scene_set_materials": (st, [vp, vp, u32]),
+        "hr_scene_set_env": (st, [vp, C.POINTER(abi.hr_env)]),
+        "hr_scene_destroy": (None, [vp]),
+        "hr_scene_info": (st, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_float)]),
+        "hr_scene_export": (st, [vp, u32, vp, sz]),
+        "hr_gbuffer_primary": (st, [vp, C.POINTER(abi.hr_camera), C.POINTER(abi.hr_gbuffer),
+                                    C.POINTER(abi.hr_stats)]),
+        "hr_trace_shadows": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
+                                  C.POINTER(abi.hr_trace_params), vp, C.POINTER(abi.hr_stats)]),
+        "hr_shade_hybrid": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
+                                 C.POINTER(abi.hr_trace_params), vp, vp, vp, C.POINTER(abi.hr_stats)]),
+        "hr_trace_rays": (st, [vp, vp, u32, i32, vp, C.POINTER(abi.hr_stats)]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.hr_context_create(device, C.byref(h))
+        if rc != 0:
+            raise HRError(rc, "hr_context_create(%d) failed (no HIP device?)" % device)
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            raise HRError(rc, self.L.hr_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hr_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(self.L.hr_synchronize(self.h))
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+
+class DeviceBuffer:
+    """A device allocation made through hr_device_alloc."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.check(ctx.L.hr_device_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.L.hr_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.L.hr_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx.check(self.ctx.L.hr_memset_d(self.ctx.h, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            self.ctx.check(self.ctx.L.hr_device_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.ctx.L.hr_device_free(self.ctx.h, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class HybridScene:
+    """A scene on the device: world-space triangles and the GPU LBVH."""
+
+    def __init__(self, ctx, scene, leaf_max=4):
+        self.ctx = ctx
+        self.scene = scene
+        self.packed = abi.PackedScene(scene)
+        bp = abi.hr_build_params(leaf_max)
+        h = C.c_void_p()
+        ctx.check(ctx.L.hr_scene_build(ctx.h, C.cast(self.packed.meshes, C.c_void_p), self.packed.mesh_count,
+                                       C.cast(self.packed.prims, C.c_void_p), self.packed.prim_count,
+                                       C.cast(self.packed.materials, C.c_void_p), self.packed.material_count,
+                                       C.byref(bp), C.byref(h)))
+        self.h = h
+        ctx.check(ctx.L.hr_scene_set_env(h, C.byref(self.packed.env)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.hr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        t, n, d, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_float()
+        self.ctx.check(self.ctx.L.hr_scene_info(self.h, C.byref(t), C.byref(n), C.byref(d), C.byref(ms)))
+        return dict(tri_count=t.value, node_count=n.value, depth=d.value, ms_build=ms.value)
+
+    def export(self, which):
+        info = self.info()
+        n = info["tri_count"]
+        size, dt = {0: (info["node_count"], abi.NODE_DTYPE), 1: (n, abi.STRI_DTYPE), 2: (n, np.uint32),
+                    3: (n, np.uint32), 4: (max(n - 1, 0) * 4, np.uint32)}[which]
+        out = np.zeros(size, dtype=dt)
+        self.ctx.check(self.ctx.L.hr_scene_export(self.h, which, out.ctypes.data, out.nbytes))
+        return out
+
+    def refit(self, models):
+        m = np.ascontiguousarray(models, dtype=np.float32)
+        self.ctx.check(self.ctx.L.hr_scene_refit(self.h, m.ctypes.data))
+
+    def set_materials(self, materials):
+        m = np.ascontiguousarray(materials, dtype=np.float32)
+        self.ctx.check(self.ctx.L.hr_scene_set_materials(self.h, m.ctypes.data, m.shape[0]))
+
+    def trace_rays(self, rays, any_hit=False, stats=False):
+        rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
+        n = len(rays)
+        rb = self.ctx.alloc(max(rays.nbytes, 32))
+        hb = self.ctx.alloc(max(n * 16, 16))
+        rb.upload(rays)
+        st = abi.hr_stats() if stats else None
+        self.ctx.check(self.ctx.L.hr_trace_rays(self.h, rb.ptr, n, int(any_hit), hb.ptr,
+                                                C.byref(st) if st is not None else None))
+        hits = hb.download(n, abi.HIT_DTYPE)
+        return (hits, st.as_dict()) if stats else hits
+
+
+PLANES = ("position", "normal", "albedo", "material", "emissive")
+
+
+class GBuffer:
+    """FP32 G-buffer planes (float4 per pixel each) in device memory."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, width, height
+        self.bufs = {k: ctx.alloc(width * height * 16) for k in PLANES}
+        self.c = abi.hr_gbuffer(width, height, *[self.bufs[k].ptr for k in PLANES])
+
+    def download(self):
+        return {k: self.bufs[k].download((self.height, self.width, 4)) for k in PLANES}
+
+    def upload(self, planes):
+        for k in PLANES:
+            self.bufs[k].upload(np.ascontiguousarray(planes[k], dtype=np.float32))
+
+
+class HybridRenderer:
+    """The hybrid pass of Renderer::rasterize_hybrid (renderer.cpp:530-600):
+    G-buffer (here: primary visibility), shadow trace (shadowRaygen), hybrid
+    trace (hybridRaygen + hybridHit). Buffers stay in device memory."""
+
+    def __init__(self, ctx, hscene, width=None, height=None):
+        self.ctx = ctx
+        self.hs = hscene
+        sc = hscene.scene
+        self.width = width or sc.width
+        self.height = height or sc.height
+        self.gb = GBuffer(ctx, self.width, self.height)
+        nl = len(sc.lights)
+        self.shadow = ctx.alloc(nl * self.width * self.height * 4)
+        self.lit = ctx.alloc(self.width * self.height * 16)
+        self.refl = ctx.alloc(self.width * self.height * 16)
+        self.set_frame(sc.camera, sc.lights, sc.shapes, sc.params)
+
+    def set_frame(self, camera, lights, shapes, params):
+        self.cam = abi.pack_camera(camera)
+        self.lights, self.shapes = abi.pack_lights(lights, shapes)
+        self.nl = len(lights)
+        self.params = abi.hr_trace_params(params["shadow_samples"], params["refl_spp"], params["max_depth"],
+                                          params.get("flags", 0))
+
+    def _shapes_ptr(self):
+        return None if self.shapes is None else C.cast(self.shapes, C.c_void_p)
+
+    def gbuffer_primary(self, stats=None):
+        L = self.ctx.L
+        self.ctx.check(L.hr_gbuffer_primary(self.hs.h, C.byref(self.cam), C.byref(self.gb.c),
+                                            C.byref(stats) if stats is not None else None))
+
+    def trace_shadows(self, stats=None):
+        L = self.ctx.L
+        self.ctx.check(L.hr_trace_shadows(self.hs.h, C.byref(self.gb.c), C.cast(self.lights, C.c_void_p),
+                                          self._shapes_ptr(), self.nl, C.byref(self.cam), C.byref(self.params),
+                                          self.shadow.ptr, C.byref(stats) if stats is not None else None))
+
+    def shade_hybrid(self, stats=None):
+        L = self.ctx.L
+        self.ctx.check(L.hr_shade_hybrid(self.hs.h, C.byref(self.gb.c), C.cast(self.lights, C.c_void_p),
+                                         self._shapes_ptr(), self.nl, C.byref(self.cam), C.byref(self.params),
+                                         self.shadow.ptr, self.lit.ptr, self.refl.ptr,
+                                         C.byref(stats) if stats is not None else None))
+
+    def rasterize_hybrid(self, stats=False):
+        """G-buffer, shadow trace, hybrid trace; with stats, returns the
+        summed ray counts and the per-pass milliseconds."""
+        if not stats:
+            self.gbuffer_primary()
+            self.trace_shadows()
+            self.shade_hybrid()
+            return None
+        parts = [abi.hr_stats() for _ in range(3)]
+        self.gbuffer_primary(parts[0])
+        self.trace_shadows(parts[1])
+        self.shade_hybrid(parts[2])
+        out = {k: sum(int(getattr(p, k)) for p in parts) for k in
+               ("primary_rays", "shadow_rays", "reflection_rays", "hit_shadow_rays", "occluded_rays",
+                "node_visits", "tri_tests")}
+        out.update(ms_gbuffer=parts[0].ms_gbuffer, ms_shadow=parts[1].ms_shadow, ms_shade=parts[2].ms_shade)
+        return out
+
+    def download_shadow(self):
+        return self.shadow.download((self.nl, self.height, self.width))
+
+    def download_lit(self):
+        return self.lit.download((self.height, self.width, 4))
+
+    def download_refl(self):
+        return self.refl.download((self.height, self.width, 4))
