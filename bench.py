@@ -35,16 +35,43 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+    this run gives the counter-to-bytes factor used for the traffic figure."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of k_shadows measured by rocprofv3 --pmc runs and
+    committed under profiles/ (None when absent)."""
+    p = os.path.join(ROOT, "profiles", "k_shadows_%s_traffic.json" % workload)
+    if not os.path.exists(p):
+        return None, None
+    d = json.load(open(p))
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(p, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hybrid_1080p_1m")
-    ap.add_argument("--cpu-rows-step", type=int, default=24)
+    ap.add_argument("--cpu-rows-step", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the hybrid pass is built and measured on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -100,8 +127,9 @@ def main():
     bytes_shadow = shadow_nodes * 64 + shadow_tris * 48 + W * H * (32 + 4 * nl)
     avg_shadow_ms = float(np.mean(ms_shadow))
     achieved = bytes_shadow / (avg_shadow_ms * 1e-3) / 1e9
+    traffic, traffic_src = load_traffic(sc.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "k_shadows", "algorithmic_bytes": int(bytes_shadow),
                 "avg_launch_ms": round(avg_shadow_ms, 3),
                 "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),
