@@ -19,6 +19,8 @@ struct hr_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     unsigned long long* d_counters = nullptr;
+    void* work = nullptr;  // grow-only workspace of the reflection pipeline
+    size_t work_bytes = 0;
     std::string err;
 };
 
@@ -228,6 +230,59 @@ hr_status frame_end(hr_context* ctx, hr_stats* stats, float* ms_slot) {
     return HR_OK;
 }
 
+// Workspace budget of one slab of the reflection pipeline; larger frames are
+// processed in slabs of pixels.
+constexpr size_t WORK_BUDGET = size_t(3) << 30;
+
+hr_status ensure_work(hr_context* ctx, size_t bytes) {
+    if (bytes <= ctx->work_bytes) return HR_OK;
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->work) {
+        HR_HIP(ctx, hipFree(ctx->work));
+        ctx->work = nullptr;
+        ctx->work_bytes = 0;
+    }
+    HR_HIP(ctx, hipMalloc(&ctx->work, bytes));
+    ctx->work_bytes = bytes;
+    return HR_OK;
+}
+
+hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
+    uint32_t spp_max = f.spp > 1 ? f.spp : 1;
+    size_t per_slot = wave_bytes_per_slot(f.nl);
+    size_t npix = (size_t)f.width * f.height;
+    size_t slab = WORK_BUDGET / (per_slot * spp_max + 32);
+    slab = slab < npix ? slab : npix;
+    slab = (slab + 255) / 256 * 256;
+    size_t slots = slab * spp_max;
+    if (slots >= (size_t(1) << 28)) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    size_t o_p0 = carve(slots * 48), o_p1 = carve(slots * 48), o_hits = carve(slots * 16);
+    size_t o_sr = carve(slots * f.nl * 32), o_vis = carve(slots * 4), o_pend = carve(slots * 4);
+    size_t o_col = carve(slots * 16), o_pix = carve(slab * 32), o_cnt = carve(64);
+    HR_TRY(ensure_work(ctx, off));
+    char* b = static_cast<char*>(ctx->work);
+    w.paths0 = reinterpret_cast<float4*>(b + o_p0);
+    w.paths1 = reinterpret_cast<float4*>(b + o_p1);
+    w.hits = reinterpret_cast<float4*>(b + o_hits);
+    w.srays = reinterpret_cast<float4*>(b + o_sr);
+    w.vis = reinterpret_cast<uint32_t*>(b + o_vis);
+    w.pending = reinterpret_cast<uint32_t*>(b + o_pend);
+    w.pathcol = reinterpret_cast<float4*>(b + o_col);
+    w.pix = reinterpret_cast<float4*>(b + o_pix);
+    w.counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    w.p0 = 0;
+    w.p1 = 0;
+    w.spp_max = spp_max;
+    w.slab_pixels = (uint32_t)slab;
+    return HR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -260,6 +315,7 @@ void hr_context_destroy(hr_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->work) (void)hipFree(ctx->work);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -657,10 +713,13 @@ hr_status hr_shade_hybrid(hr_scene* s, const hr_gbuffer* gb, const hr_light* lig
     FrameArgs f;
     HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
     HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
+    if (params->refl_spp > 64) return fail(ctx, HR_E_UNSUPPORTED, "refl_spp above 64");
     bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    WaveArgs w;
+    HR_TRY(wave_args(ctx, f, w));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, launch_shade(dev_scene(s), f, gview(gb), shadow, reinterpret_cast<float4*>(lit),
-                             reinterpret_cast<float4*>(refl), ctx->d_counters, count, ctx->stream));
+                             reinterpret_cast<float4*>(refl), ctx->d_counters, count, ctx->stream, w));
     return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 

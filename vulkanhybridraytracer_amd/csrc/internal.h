@@ -99,8 +99,24 @@ hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, un
                           bool count, hipStream_t st);
 hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
                           unsigned long long* counters, bool count, hipStream_t st);
+// Workspace of the wavefront reflection pipeline for one slab of pixels
+// [p0, p1): path queues (ping-pong), hits, hit shadow rays, visibility masks,
+// pending list, per (pixel, sample) results and per-pixel direct / ambient.
+struct WaveArgs {
+    float4* paths0;
+    float4* paths1;
+    float4* hits;
+    float4* srays;
+    uint32_t* vis;
+    uint32_t* pending;
+    float4* pathcol;
+    float4* pix;
+    uint32_t* counts;  // [0], [1] path queues, [2] shadow rays, [3] pending
+    uint32_t p0, p1, spp_max, slab_pixels;
+};
+size_t wave_bytes_per_slot(uint32_t nl);
 hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, const float* shadow, float4* lit,
-                        float4* refl, unsigned long long* counters, bool count, hipStream_t st);
+                        float4* refl, unsigned long long* counters, bool count, hipStream_t st, WaveArgs w);
 hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits,
                              unsigned long long* counters, bool count, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);

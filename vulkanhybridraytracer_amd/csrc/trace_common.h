@@ -1,0 +1,248 @@
+// trace_common.h - traversal, texture, BRDF and sampling device functions
+// shared by the trace kernels (trace.hip) and the wavefront reflection
+// pipeline (wavefront.hip).
+#pragma once
+
+#include "device_math.h"
+#include "internal.h"
+
+namespace hr {
+namespace tc {
+using namespace hrd;
+
+
+constexpr int BLOCK = 256;
+constexpr float PI_H = 3.14159265359f;  // helpers.glsl:5
+constexpr float PI_M = 3.141592f;       // miss.rmiss:11
+
+struct Cnt {
+    uint32_t shadow, refl, hitshadow, occluded, nodes, tris, primary, overflow;
+};
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    uint32_t xcd = b & 7u, slot = b >> 3;
+    uint32_t per = n >> 3, rem = n & 7u;
+    return xcd * per + min(xcd, rem) + slot;
+}
+__device__ __forceinline__ void pixel_of(uint32_t W, uint32_t& x, uint32_t& y) {
+    uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    uint32_t tx = (W + 15u) / 16u;
+    uint32_t bx = bid % tx, by = bid / tx;
+    uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    x = bx * 16u + (w & 1u) * 8u + (lane & 7u);
+    y = by * 16u + (w >> 1) * 8u + (lane >> 3);
+}
+
+struct HitD {
+    float t, u, v;
+    uint32_t tri;
+};
+
+template <bool ANY, bool COUNT>
+__device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tmax, uint32_t* lds, HitD& h,
+                         Cnt& c) {
+    if (S.node_count == 0) return false;
+    uint32_t spill[STACK_SPILL];
+    int sp = 0;
+    uint32_t cur = 0;
+    float tbest = tmax;
+    uint32_t best = 0xFFFFFFFFu;
+    float bu = 0.0f, bv = 0.0f;
+    for (;;) {
+        if (!(cur & LEAF_BIT)) {
+            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 a = np[0], b = np[1], cc = np[2], d = np[3];
+            if (COUNT) ++c.nodes;
+            float tn0, tn1;
+            bool h0 = box_hit(a.x, a.y, a.z, b.x, b.y, b.z, r, tmin, tbest, tn0);
+            bool h1 = box_hit(cc.x, cc.y, cc.z, d.x, d.y, d.z, r, tmin, tbest, tn1);
+            uint32_t c0 = __float_as_uint(a.w), c1 = __float_as_uint(b.w);
+            if (h0 && h1) {
+                uint32_t nearc = c0, farc = c1;
+                if (tn1 < tn0) {
+                    nearc = c1;
+                    farc = c0;
+                }
+                if (sp < (int)STACK_LDS) {
+                    lds[sp * BLOCK] = farc;
+                    ++sp;
+                } else if (sp < (int)MAX_DEPTH) {
+                    spill[sp - STACK_LDS] = farc;
+                    ++sp;
+                } else {
+                    c.overflow = 1;
+                }
+                cur = nearc;
+                continue;
+            }
+            if (h0) {
+                cur = c0;
+                continue;
+            }
+            if (h1) {
+                cur = c1;
+                continue;
+            }
+        } else {
+            uint32_t first = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                if (COUNT) ++c.tris;
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tbest, t, b1, b2)) {
+                    uint32_t id = __float_as_uint(v0.w);
+                    if (ANY) {
+                        h.t = t;
+                        h.u = b1;
+                        h.v = b2;
+                        h.tri = id;
+                        return true;
+                    }
+                    if (t < tbest || id < best) {
+                        tbest = t;
+                        best = id;
+                        bu = b1;
+                        bv = b2;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+    }
+    if (best == 0xFFFFFFFFu) return false;
+    h.t = tbest;
+    h.u = bu;
+    h.v = bv;
+    h.tri = best;
+    return true;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ bool occluded(const DevScene& S, v3 o, v3 d, float tmin, float tmax, uint32_t* lds,
+                                         Cnt& c) {
+    RayP r = prep_ray(o, d);
+    HitD h;
+    return traverse<true, COUNT>(S, r, tmin, tmax, lds, h, c);
+}
+
+// ---- textures ---------------------------------------------------------------
+__device__ __forceinline__ int wrap_index(float f, uint32_t n) {
+    int i = (f == f && fabsf(f) < 1.0e9f) ? (int)f : 0;
+    i %= (int)n;
+    if (i < 0) i += (int)n;
+    return i;
+}
+__device__ __forceinline__ v3 tex_nearest(const float4* t, uint32_t w, uint32_t h, float u, float v) {
+    if (w == 0 || h == 0) return mk(0.0f, 0.0f, 0.0f);
+    int x = wrap_index(floorf(u * (float)w), w);
+    int y = wrap_index(floorf(v * (float)h), h);
+    return ld3(t[(size_t)y * w + (size_t)x]);
+}
+__device__ __forceinline__ v3 env_lookup(const float4* t, uint32_t w, uint32_t h, v3 d, float pi) {
+    float u = 0.5f + atan2_hr(d.x, d.z) / (2.0f * pi);
+    float v = 0.5f - asin_hr(d.y) / pi;
+    return tex_nearest(t, w, h, u, v);
+}
+
+// ---- helpers.glsl -----------------------------------------------------------
+__device__ __forceinline__ float DistributionGGX(v3 N, v3 H, float roughness) {
+    float a = roughness * roughness;
+    float a2 = a * a;
+    float NdotH = fmaxf(dot(N, H), 0.0f);
+    float NdotH2 = NdotH * NdotH;
+    float denom = NdotH2 * (a2 - 1.0f) + 1.0f;
+    denom = PI_H * denom * denom;
+    return a2 / denom;
+}
+__device__ __forceinline__ float GeometrySchlickGGX(float NdotV, float roughness) {
+    float r = roughness + 1.0f;
+    float k = (r * r) / 8.0f;
+    float denom = NdotV * (1.0f - k) + k;
+    return NdotV / denom;
+}
+__device__ __forceinline__ float GeometrySmith(v3 N, v3 V, v3 L, float roughness) {
+    float NdotV = fmaxf(dot(N, V), 0.0f);
+    float NdotL = fmaxf(dot(N, L), 0.0f);
+    return GeometrySchlickGGX(NdotV, roughness) * GeometrySchlickGGX(NdotL, roughness);
+}
+__device__ __forceinline__ v3 FresnelSchlick(float cosTheta, v3 F0) {
+    float p = powf(1.0f - cosTheta, 5.0f);
+    return mk(F0.x + (1.0f - F0.x) * p, F0.y + (1.0f - F0.y) * p, F0.z + (1.0f - F0.z) * p);
+}
+__device__ __forceinline__ v3 sampleDisk(const hr_light& light, v3 position, v3 L, float r1, float r2) {
+    float radius = light.radius * sqrtf(r1);
+    float sn, cs;
+    sincos2pi_hr(r2, &sn, &cs);
+    float px = radius * cs, py = radius * sn;
+    v3 tangent = normalize(cross(L, mk(0.0f, 1.0f, 0.0f)));
+    if (L.x == 0.0f && L.y == 1.0f && L.z == 0.0f) tangent = mk(0.0f, 0.0f, 1.0f);
+    v3 bitangent = normalize(cross(tangent, L));
+    v3 target = add(add(add(position, L), scl(tangent, px)), scl(bitangent, py));
+    return normalize(sub(target, position));
+}
+__device__ __forceinline__ v3 sampleRect(v3 centre, const hr_light_shape& sh, v3 position, uint32_t s, uint32_t S,
+                                         float r1, float r2) {
+    uint32_t n = 1;
+    while ((n + 1) * (n + 1) <= S) ++n;
+    uint32_t m = (S + n - 1) / n;
+    float fu = ((float)(s % n) + r1) / (float)n - 0.5f;
+    float fv = ((float)(s / n) + r2) / (float)m - 0.5f;
+    v3 eu = ld3p(sh.edge_u), ev = ld3p(sh.edge_v);
+    v3 target = add(add(centre, scl(eu, fu)), scl(ev, fv));
+    return normalize(sub(target, position));
+}
+__device__ __forceinline__ void createCoordSystem(v3 N, v3& T, v3& B) {  // random.glsl:46-53
+    if (fabsf(N.x) > fabsf(N.y))
+        T = divs(mk(N.z, 0.0f, -N.x), sqrtf(N.x * N.x + N.z * N.z));
+    else
+        T = divs(mk(0.0f, -N.z, N.y), sqrtf(N.y * N.y + N.z * N.z));
+    B = cross(N, T);
+}
+__device__ __forceinline__ v3 sampleGGX(v3 N, float roughness, float r1, float r2) {
+    float a = roughness * roughness;
+    float cos2 = (1.0f - r1) / (1.0f + (a * a - 1.0f) * r1);
+    float ct = sqrtf(cos2);
+    float st = sqrtf(fmaxf(1.0f - cos2, 0.0f));
+    float sn, cs;
+    sincos2pi_hr(r2, &sn, &cs);
+    v3 T, B;
+    createCoordSystem(N, T, B);
+    return add(add(scl(T, st * cs), scl(B, st * sn)), scl(N, ct));
+}
+__device__ __forceinline__ v3 light_sample_dir(const FrameArgs& F, uint32_t i, v3 P, v3 L, Pcg& g, uint32_t s,
+                                               uint32_t S) {
+    const hr_light& lt = F.lights[i];
+    float r1 = u01(g);
+    float r2 = u01(g);
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    if (!isDir && F.has_shapes && F.shapes[i].type == HR_LIGHT_RECT)
+        return sampleRect(ld3p(lt.pos_maxdist), F.shapes[i], P, s, S, r1, r2);
+    return sampleDisk(lt, P, L, r1, r2);
+}
+__device__ __forceinline__ int mat_index(const DevScene& S, float w) {
+    int i = (int)w;
+    return (i < 0 || i >= (int)S.nmat) ? 0 : i;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    return v;
+}
+__device__ __forceinline__ void flush_counters(const Cnt& c, unsigned long long* counters, bool count) {
+    uint32_t v[8] = {c.primary, c.shadow, c.refl, c.hitshadow, c.occluded, c.nodes, c.tris, c.overflow};
+    const int slot[8] = {C_PRIMARY, C_SHADOW, C_REFL, C_HITSHADOW, C_OCCLUDED, C_NODES, C_TRIS, C_OVERFLOW};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!count && (k == 5 || k == 6)) continue;
+        uint32_t s = wave_sum(v[k]);
+        if ((threadIdx.x & 63u) == 0 && s) atomicAdd(&counters[slot[k]], (unsigned long long)s);
+    }
+}
+
+}  // namespace tc
+}  // namespace hr
