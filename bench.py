@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--workload", default="hybrid_1080p_1m")
     ap.add_argument("--cpu-rows-step", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--leaf-max", type=int, default=4, help="largest subtree stored as one BVH leaf")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
@@ -84,7 +85,7 @@ def main():
     log("scene %s: %d triangles, %dx%d, generated in %.1fs" % (sc.name, sc.tri_count, sc.width, sc.height,
                                                               time.time() - t0))
     ctx = hr.Context(0)
-    hs = hr.HybridScene(ctx, sc)
+    hs = hr.HybridScene(ctx, sc, leaf_max=args.leaf_max)
     info = hs.info()
     log("LBVH build %.2f ms, depth %d" % (info["ms_build"], info["depth"]))
     rd = hr.HybridRenderer(ctx, hs)
@@ -158,7 +159,7 @@ def main():
            "data": "synthetic",
            "config": {"workload": sc.name, "width": W, "height": H, "triangles": sc.tri_count, "lights": nl,
                       "shadow_samples": sc.params["shadow_samples"], "refl_spp": sc.params["refl_spp"],
-                      "max_depth": sc.params["max_depth"]},
+                      "max_depth": sc.params["max_depth"], "leaf_max": args.leaf_max},
            "rays_per_step": per_step, "ms_shadow": round(avg_shadow_ms, 3),
            "ms_shade": round(float(np.mean(ms_shade)), 3), "ms_gbuffer": round(gst["ms_gbuffer"], 3),
            "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu}

@@ -19,6 +19,8 @@ struct hr_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     unsigned long long* d_counters = nullptr;
+    uint32_t* d_next_group = nullptr;  // work counter of the persistent shadow kernel
+    uint32_t resident_blocks = 2048;    // 256-thread workgroups resident at once
     void* work = nullptr;  // grow-only workspace of the reflection pipeline
     size_t work_bytes = 0;
     std::string err;
@@ -302,6 +304,12 @@ hr_status hr_context_create(int device, hr_context** out) {
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[1]);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_next_group, 64);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            ctx->resident_blocks = (uint32_t)prop.multiProcessorCount * 8u;  // 8 x 4 waves per CU
+    }
     if (e != hipSuccess) {
         hr_context_destroy(ctx);
         return e == hipErrorOutOfMemory ? HR_E_OOM : HR_E_DEVICE;
@@ -315,6 +323,7 @@ void hr_context_destroy(hr_context* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_next_group) (void)hipFree(ctx->d_next_group);
     if (ctx->work) (void)hipFree(ctx->work);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -699,7 +708,8 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
     bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
     HR_TRY(frame_begin(ctx, stats != nullptr));
-    HR_HIP(ctx, launch_shadows(dev_scene(s), f, gview(gb), shadow, ctx->d_counters, count, ctx->stream));
+    HR_HIP(ctx, launch_shadows(dev_scene(s), f, gview(gb), shadow, ctx->d_counters, count, ctx->stream,
+                               ctx->d_next_group, ctx->resident_blocks));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 

@@ -97,8 +97,11 @@ enum : int {
 // ---- launchers (trace.hip) ---------------------------------------------------
 hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, unsigned long long* counters,
                           bool count, hipStream_t st);
+// next_group: one device uint32 (reset by the launch); resident_blocks: the
+// persistent grid size (workgroups that fit on the device at once)
 hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
-                          unsigned long long* counters, bool count, hipStream_t st);
+                          unsigned long long* counters, bool count, hipStream_t st, uint32_t* next_group,
+                          uint32_t resident_blocks);
 // Workspace of the wavefront reflection pipeline for one slab of pixels
 // [p0, p1): path queues (ping-pong), hits, hit shadow rays, visibility masks,
 // pending list, per (pixel, sample) results and per-pixel direct / ambient.

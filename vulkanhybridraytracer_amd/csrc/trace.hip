@@ -81,55 +81,166 @@ __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBuf
     flush_counters(c, counters, COUNT);
 }
 
+// Shadow pass, persistent waves with dynamic fetch. The work unit is a
+// group = (pixel, light): its `samples` rays are traced by one lane in
+// sample order and its shadow factor is written once by that lane. Groups are
+// numbered tile by tile (8x8 pixels) and light by light, so the 64 groups a
+// wave takes together are one light's rays from one 8x8 tile. A lane whose
+// group is finished takes the next group from a global counter (one atomic
+// per wave for all lanes that need work), so lanes stay busy until the frame
+// runs out of groups instead of idling beside pixels with more rays.
+// Traversal: near-first stack walk, internal nodes in an inner loop until the
+// lane reaches a leaf, then the leaf's triangles.
 template <bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBufView G, float* shadow,
+                                                   uint32_t* next_group, uint32_t ngroups,
                                                    unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     uint32_t* lds = stk + threadIdx.x;
+    constexpr uint32_t DONE = 0xFFFFFFFFu;  // no leaf code reaches it (fewer than 2^27 triangles)
     Cnt c = {};
-    uint32_t x, y;
-    pixel_of(F.width, x, y);
-    if (x < F.width && y < F.height) {
-        size_t plane = (size_t)F.width * F.height;
-        size_t p = (size_t)y * F.width + x;
-        uint32_t seed = tea(y * F.width + x, F.frame);
-        float4 P4 = G.pos[p], N4 = G.nrm[p];
-        bool background = P4.w == 0.0f && N4.w == 0.0f;
-        int mode = (int)S.mats[mat_index(S, P4.w)].shading_metal_rough_idx[0];
-        if (background || mode == 4) {
-            float v = background ? 0.0f : 1.0f;
-            for (uint32_t i = 0; i < F.nl; ++i) shadow[i * plane + p] = v;
-        } else {
-            v3 P = ld3(P4);
-            v3 N = normalize(ld3(N4));
-            uint32_t SS = F.samples;
-            for (uint32_t i = 0; i < F.nl; ++i) {
-                const hr_light& lt = F.lights[i];
-                bool isDir = lt.pos_maxdist[3] < 0.0f;
-                v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
-                float maxd = lt.pos_maxdist[3];
-                float dist = length(L);
-                L = normalize(L);
-                float NdotL = clamp01(dot(N, L));
-                float factor = 0.0f;
-                if (NdotL > 0.0f) {
-                    for (uint32_t s = 0; s < SS; ++s) {
-                        bool shadowed;
-                        if (isDir || dist < maxd) {
-                            Pcg g = ray_stream(seed, PURPOSE_SURFACE_SHADOW, i * SS + s);
-                            v3 dir = light_sample_dir(F, i, P, L, g, s, SS);
-                            float tmax = isDir ? 1000.0f : dist + 1.0f;
-                            ++c.shadow;
-                            shadowed = occluded<COUNT>(S, add(P, scl(dir, 1e-1f)), dir, 0.001f, tmax, lds, c);
-                            if (shadowed) ++c.occluded;
-                        } else {
-                            shadowed = false;
-                        }
-                        if (!shadowed) factor += 1.0f;
-                    }
-                    factor /= (float)SS;
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const size_t plane = (size_t)F.width * F.height;
+    const uint32_t SS = F.samples;
+    const uint32_t tiles_x = (F.width + 7u) / 8u;
+    const uint32_t per_tile = F.nl * 64u;
+    // current group
+    size_t p = 0;
+    uint32_t li = 0, seed = 0, s = 0, unocc = 0;
+    v3 P = mk(0.0f, 0.0f, 0.0f), L = P;
+    float dist = 0.0f;
+    bool isDir = false, in_group = false;
+    // current ray
+    uint32_t cur = DONE;
+    bool hit = false;
+    RayP r;
+    float tmax = 0.0f;
+    const float tmin = 0.001f;
+    int sp = 0;
+    uint32_t spill[STACK_SPILL];
+    for (;;) {
+        if (cur == DONE) {
+            if (in_group) {
+                if (s > 0) {  // a ray of this group just finished
+                    if (hit) ++c.occluded; else ++unocc;
                 }
-                shadow[i * plane + p] = factor;
+                if (s == SS) {
+                    shadow[li * plane + p] = (float)unocc / (float)SS;
+                    in_group = false;
+                }
+            }
+            if (!in_group) {
+                // take groups until one has rays to trace
+                bool exhausted = false;
+                for (;;) {
+                    unsigned long long need = __ballot(1);
+                    uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+                    uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(next_group, (uint32_t)__popcll(need));
+                    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+                    uint32_t g = base + rank;
+                    if (g >= ngroups) {
+                        exhausted = true;
+                        break;
+                    }
+                    uint32_t t = g / per_tile, rem = g - t * per_tile;
+                    li = rem >> 6;
+                    uint32_t q = rem & 63u;
+                    uint32_t x = (t % tiles_x) * 8u + (q & 7u), y = (t / tiles_x) * 8u + (q >> 3);
+                    if (x >= F.width || y >= F.height) continue;
+                    p = (size_t)y * F.width + x;
+                    float4 P4 = G.pos[p], N4 = G.nrm[p];
+                    bool background = P4.w == 0.0f && N4.w == 0.0f;
+                    int mode = (int)S.mats[mat_index(S, P4.w)].shading_metal_rough_idx[0];
+                    if (background || mode == 4) {
+                        shadow[li * plane + p] = background ? 0.0f : 1.0f;
+                        continue;
+                    }
+                    P = ld3(P4);
+                    v3 N = normalize(ld3(N4));
+                    const hr_light& lt = F.lights[li];
+                    isDir = lt.pos_maxdist[3] < 0.0f;
+                    L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
+                    dist = length(L);
+                    L = normalize(L);
+                    float NdotL = clamp01(dot(N, L));
+                    if (!(NdotL > 0.0f)) {
+                        shadow[li * plane + p] = 0.0f;
+                        continue;
+                    }
+                    if (!(isDir || dist < lt.pos_maxdist[3])) {
+                        shadow[li * plane + p] = 1.0f;
+                        continue;
+                    }
+                    seed = tea(y * F.width + x, F.frame);
+                    s = 0;
+                    unocc = 0;
+                    in_group = true;
+                    break;
+                }
+                if (exhausted) break;
+            }
+            Pcg g = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
+            v3 dir = light_sample_dir(F, li, P, L, g, s, SS);
+            tmax = isDir ? 1000.0f : dist + 1.0f;
+            r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
+            ++c.shadow;
+            ++s;
+            hit = false;
+            sp = 0;
+            cur = S.node_count ? 0u : DONE;
+            continue;
+        }
+        while (cur < LEAF_BIT) {
+            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 a = np[0], b = np[1], cc = np[2], d = np[3];
+            if (COUNT) ++c.nodes;
+            float tn0, tn1;
+            bool h0 = box_hit(a.x, a.y, a.z, b.x, b.y, b.z, r, tmin, tmax, tn0);
+            bool h1 = box_hit(cc.x, cc.y, cc.z, d.x, d.y, d.z, r, tmin, tmax, tn1);
+            uint32_t c0 = __float_as_uint(a.w), c1 = __float_as_uint(b.w);
+            if (h0 && h1) {
+                uint32_t nearc = c0, farc = c1;
+                if (tn1 < tn0) {
+                    nearc = c1;
+                    farc = c0;
+                }
+                if (sp < (int)STACK_LDS) lds[sp * BLOCK] = farc;
+                else if (sp < (int)MAX_DEPTH) spill[sp - STACK_LDS] = farc;
+                else c.overflow = 1;
+                sp = min(sp + 1, (int)MAX_DEPTH);
+                cur = nearc;
+            } else if (h0) {
+                cur = c0;
+            } else if (h1) {
+                cur = c1;
+            } else if (sp == 0) {
+                cur = DONE;
+            } else {
+                --sp;
+                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+            }
+        }
+        if (cur != DONE) {
+            uint32_t first = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                if (COUNT) ++c.tris;
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2)) {
+                    hit = true;
+                    break;
+                }
+            }
+            if (hit || sp == 0) {
+                cur = DONE;
+            } else {
+                --sp;
+                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
             }
         }
     }
@@ -166,6 +277,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DevScene S, const hr_ray* rays,
     flush_counters(c, counters, COUNT);
 }
 
+__global__ void k_reset_u32(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -186,9 +301,14 @@ hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, un
     return hipGetLastError();
 }
 hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
-                          unsigned long long* counters, bool count, hipStream_t st) {
-    if (count) k_shadows<true><<<pixel_blocks(f), BLOCK, 0, st>>>(s, f, gb, shadow, counters);
-    else k_shadows<false><<<pixel_blocks(f), BLOCK, 0, st>>>(s, f, gb, shadow, counters);
+                          unsigned long long* counters, bool count, hipStream_t st, uint32_t* next_group,
+                          uint32_t resident_blocks) {
+    uint32_t tiles = ((f.width + 7u) / 8u) * ((f.height + 7u) / 8u);
+    uint32_t ngroups = tiles * f.nl * 64u;
+    uint32_t nb = min(resident_blocks, (ngroups + BLOCK - 1) / BLOCK);
+    k_reset_u32<<<1, 64, 0, st>>>(next_group);
+    if (count) k_shadows<true><<<nb, BLOCK, 0, st>>>(s, f, gb, shadow, next_group, ngroups, counters);
+    else k_shadows<false><<<nb, BLOCK, 0, st>>>(s, f, gb, shadow, next_group, ngroups, counters);
     return hipGetLastError();
 }
 hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits,
