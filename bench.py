@@ -94,7 +94,7 @@ def main():
 
     # counting launches (node / triangle visits) outside the timed region
     flags_saved = rd.params.flags
-    rd.params.flags = 1
+    rd.params.flags = flags_saved | 1
     from vulkanhybridraytracer_amd import abi
     cs = abi.hr_stats()
     rd.trace_shadows(cs)

@@ -178,7 +178,7 @@ typedef struct hr_gbuffer {
  *   (1 in the reference).
  * max_depth: bounce limit (MAX_RECURSION = 8, helpers.glsl:8). */
 typedef struct hr_trace_params {
-    uint32_t shadow_samples;
+    uint32_t shadow_samples;  /* unused with HR_FLAG_ADAPTIVE */
     uint32_t refl_spp;
     uint32_t max_depth;
     uint32_t flags;
@@ -221,7 +221,15 @@ typedef struct hr_stats {
 } hr_stats;
 
 #define HR_FLAG_COUNT_VISITS 1u  /* count node visits / triangle tests (slower kernel variant) */
-#define HR_FLAG_TIMING 2u        /* record HIP events around each kernel */
+#define HR_FLAG_TIMING 2u        /* reserved */
+/* Adaptive sampling (no counterpart in the reference, whose sample count is a
+ * uniform, renderer.cpp:656):
+ *  - shadows: a disk light of radius 0 takes 1 ray; other lights take 4
+ *    stratified probe rays and, when the probes disagree (a penumbra), 64
+ *    stratified rays that replace them; shadow_samples is then unused;
+ *  - reflections: a glossy pixel (shading model 3, roughness r > 0) traces
+ *    1 + floor((refl_spp - 1) * r) paths. */
+#define HR_FLAG_ADAPTIVE 4u
 
 typedef struct hr_build_params {
     uint32_t leaf_max;   /* largest Karras subtree stored as one leaf (1..8, default 4) */

@@ -220,7 +220,8 @@ enum : uint32_t {
     PURPOSE_SURFACE_SHADOW = 1, /* index = light * samples + sample */
     PURPOSE_REFLECT = 2,        /* index = spp sample (GGX lobe at the G-buffer surface) */
     PURPOSE_HIT_SHADOW = 3,     /* index = (sample * max_depth + depth) * nl + light */
-    PURPOSE_HIT_REFLECT = 4     /* index = sample * max_depth + depth */
+    PURPOSE_HIT_REFLECT = 4,    /* index = sample * max_depth + depth */
+    PURPOSE_ADAPTIVE_SHADOW = 5 /* index = light * 128 + k (k < 4 probes, 64 + s refinement) */
 };
 static inline Pcg ray_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index) {
     return pcg_init((uint64_t)pixel_seed, ((uint64_t)purpose << 32) | (uint64_t)index);
@@ -882,7 +883,37 @@ struct Frame {
     uint32_t samples, spp, max_depth;
     uint32_t W, H;
     uint32_t frame;
+    bool adaptive;
 };
+/* adaptive sampling: cell s of the n x ceil(S/n) grid on the light */
+static inline v3 light_sample_dir_strat(const Frame& f, uint32_t i, v3 P, v3 L, Pcg& g, uint32_t s, uint32_t S) {
+    const hr_light& lt = f.lights[i];
+    float r1 = u01(g);
+    float r2 = u01(g);
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    if (!isDir && f.shapes && f.shapes[i].type == HR_LIGHT_RECT)
+        return sampleRect(mk(lt.pos_maxdist[0], lt.pos_maxdist[1], lt.pos_maxdist[2]), f.shapes[i], P, s, S, r1, r2);
+    uint32_t n = 1;
+    while ((n + 1) * (n + 1) <= S) ++n;
+    uint32_t m = (S + n - 1) / n;
+    float a = ((float)(s % n) + r1) / (float)n;
+    float b = ((float)(s / n) + r2) / (float)m;
+    return sampleDisk(lt, P, L, a, b);
+}
+static inline bool hard_light(const Frame& f, uint32_t i) {
+    const hr_light& lt = f.lights[i];
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    bool rect = !isDir && f.shapes && f.shapes[i].type == HR_LIGHT_RECT;
+    return !rect && lt.radius == 0.0f;
+}
+/* glossy paths per pixel (adaptive: 1 + floor((refl_spp - 1) * roughness)) */
+static inline uint32_t pixel_spp(const Frame& f, int mode, float roughness) {
+    if (!(mode == 3 && roughness > 0.0f)) return 1u;
+    uint32_t m = std::max(f.spp, 1u);
+    if (!f.adaptive) return m;
+    uint32_t k = 1u + (uint32_t)floorf((float)(m - 1u) * roughness);
+    return std::min(k, m);
+}
 static inline uint32_t frame_index(const hr_camera* cam) { return (uint32_t)(int)cam->frame[0]; }
 static inline int mat_index(const or_scene& s, float w) {
     int i = (int)w;
@@ -1257,7 +1288,8 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
                       const hr_trace_params* prm, float* shadow, uint32_t row_start, uint32_t row_step,
                       or_counters* cnt, uint32_t threads) {
     Cnt total;
-    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam)};
+    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam),
+            (prm->flags & HR_FLAG_ADAPTIVE) != 0};
     size_t plane = (size_t)W * H;
     parallel_rows(H, row_start, row_step, threads, [&](uint32_t y, Cnt& c) {
         for (uint32_t x = 0; x < W; ++x) {
@@ -1287,7 +1319,29 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
                 L = normalize(L);
                 float NdotL = clamp01(dot(N, L));
                 float factor = 0.0f;
-                if (NdotL > 0.0f) {
+                if (NdotL > 0.0f && f.adaptive && (isDir || dist < maxd)) {
+                    /* 1 ray for hard lights, else 4 probes; 64 rays when they disagree */
+                    uint32_t target = hard_light(f, i) ? 1u : 4u;
+                    bool refine = false;
+                    for (;;) {
+                        uint32_t unocc = 0;
+                        for (uint32_t sI = 0; sI < target; ++sI) {
+                            Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, i * 128u + (refine ? 64u + sI : sI));
+                            v3 dir = light_sample_dir_strat(f, i, P, L, g, sI, target);
+                            float tmax = isDir ? 1000.0f : dist + 1.0f;
+                            ++c.shadow;
+                            bool sh = occluded(*s, add(P, scl(dir, 1e-1f)), dir, 0.001f, tmax, c);
+                            if (sh) ++c.occluded; else ++unocc;
+                        }
+                        if (!refine && target == 4u && unocc != 0u && unocc != 4u) {
+                            refine = true;
+                            target = 64u;
+                            continue;
+                        }
+                        factor = (float)unocc / (float)target;
+                        break;
+                    }
+                } else if (NdotL > 0.0f) {
                     for (uint32_t sI = 0; sI < S; ++sI) {
                         bool shadowed;
                         if (isDir || dist < maxd) {
@@ -1318,7 +1372,8 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
                      const float* shadow, float* lit, float* refl, uint32_t row_start, uint32_t row_step,
                      or_counters* cnt, uint32_t threads) {
     Cnt total;
-    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam)};
+    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam),
+            (prm->flags & HR_FLAG_ADAPTIVE) != 0};
     size_t plane = (size_t)W * H;
     float cpos[4];
     mul4(cam->inv_view, 0.0f, 0.0f, 0.0f, 1.0f, cpos);
@@ -1410,7 +1465,7 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
             }
             v3 rfac = mk(1, 1, 1);
             if ((mode == 3 || mode == 4) && f.max_depth > 0) {
-                uint32_t spp = (mode == 3 && roughness > 0.0f) ? std::max(f.spp, 1u) : 1u;
+                uint32_t spp = pixel_spp(f, mode, roughness);
                 v3 acc = mk(0, 0, 0);
                 for (uint32_t sI = 0; sI < spp; ++sI) {
                     v3 d = direction;

@@ -171,7 +171,7 @@ class OracleScene:
         p = dict(sc.params)
         if params:
             p.update(params)
-        return abi.hr_trace_params(p["shadow_samples"], p["refl_spp"], p["max_depth"], 0)
+        return abi.hr_trace_params(p["shadow_samples"], p["refl_spp"], p["max_depth"], p.get("flags", 0) & ~1)
 
     def trace_shadows(self, gb, camera=None, params=None, rows=(0, 1), lights=None, shapes="scene"):
         sc = self.scene

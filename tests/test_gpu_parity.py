@@ -149,6 +149,34 @@ def test_gbuffer_shadow_shade_small(gpu_ctx, oracle_lib):
     assert st["hit_shadow_rays"] == hc["hit_shadow_rays"]
 
 
+def test_adaptive_parity(gpu_ctx, oracle_lib):
+    """4M-generator settings (adaptive 1/4/64 shadow rays per light, 1-64
+    glossy paths by roughness) at 320x180 on a smaller scene: every 4th row."""
+    hr = _hr()
+    sc = scenes.make_terrain_scene(seed=1, width=320, height=180, quads=128, instances=48, samples=64,
+                                   spp=64, rough_max=1.0, adaptive=True)
+    hs, os_ = _pair(gpu_ctx, oracle_lib, sc)
+    rd = hr.HybridRenderer(gpu_ctx, hs)
+    st = rd.rasterize_hybrid(stats=True)
+    gb = rd.gb.download()
+    rows = (1, 4)
+    sel = slice(rows[0], None, rows[1])
+    sh = rd.download_shadow()
+    osh, oc = os_.trace_shadows(gb, rows=rows)
+    frac, _ = _mismatch_frac(sh[:, sel], osh[:, sel])
+    print("adaptive shadow mismatch", frac, "soft", float(((sh > 0) & (sh < 1)).mean()))
+    assert frac < GRAZE_FRAC
+    assert ((sh > 0) & (sh < 1)).any()
+    lit = rd.download_lit()
+    olit, orefl, hc = os_.shade_hybrid(gb, sh, rows=rows)
+    d = np.nan_to_num(np.abs(lit[sel] - olit[sel]))
+    print("adaptive lit max abs err", d.max())
+    assert ((d > RAD_TOL).any(-1)).mean() < GRAZE_FRAC
+    # the full-frame GPU counts equal the oracle's on the full frame
+    _, oc_full = os_.trace_shadows(gb)
+    assert st["shadow_rays"] == oc_full["shadow_rays"]
+
+
 def test_terrain_rows_parity(gpu_ctx, oracle_lib):
     """1080p-generator scene at 480x270 with all features (rect lights, 16
     stratified samples, GGX 4 spp): every 8th row against the oracle."""

@@ -151,6 +151,39 @@ def test_frame_index_changes_stream_only_for_sampled_lights(oracle_lib):
     assert np.all(np.isin(vals, np.arange(9) / 8.0))  # fractions of 8 samples
 
 
+def test_adaptive_sampling_counts(oracle_lib):
+    """Adaptive sampling: per (pixel, light) 4 probe rays, then 64 more when
+    the probes disagree, so factors are 0, 1 or k/64 and the ray count is
+    4 * groups + 64 * penumbra groups; glossy pixels trace 1 + floor(63 r)
+    reflection paths."""
+    sc = scenes.make_terrain_scene(seed=4, width=80, height=45, quads=48, instances=24, soup_tris=128,
+                                   samples=64, spp=64, rough_max=1.0, adaptive=True)
+    os_ = oracle_lib.OracleScene(sc)
+    gb, _ = os_.gbuffer()
+    sh, cnt = os_.trace_shadows(gb)
+    vals = np.unique(sh)
+    assert np.all(np.isin(vals, np.arange(65, dtype=np.float32) / np.float32(64)))
+    soft = (sh > 0) & (sh < 1)
+    assert soft.sum() > 0
+    # rays: every traced group costs 4; penumbra groups 64 more. Groups whose
+    # light faces away or is the background cost nothing (factor 0).
+    lit_or_shadowed = (sh == 1) | (sh == 0)
+    assert cnt["shadow_rays"] >= 64 * int(soft.sum()) + 4 * int(soft.sum())
+    assert (cnt["shadow_rays"] - 68 * int(soft.sum())) % 4 == 0
+    # non-adaptive at 64 samples traces more rays for the same scene
+    _, c64 = os_.trace_shadows(gb, params=dict(flags=0, shadow_samples=64))
+    assert c64["shadow_rays"] > cnt["shadow_rays"]
+    lit, refl, c2 = os_.shade_hybrid(gb, sh)
+    rough = gb["material"][..., 1]
+    glossy = (gb["normal"][..., 3] == 1) & (rough > 0)
+    mat_idx = gb["position"][..., 3].astype(int)
+    modes = sc.materials[mat_idx, 8]
+    glossy &= modes == 3
+    want_max = int(np.sum(1 + np.floor(np.float32(63) * rough[glossy]).astype(np.int64)))
+    assert 0 < c2["reflection_rays"] <= want_max
+    assert np.isfinite(lit).all() and lit_or_shadowed.sum() > 0
+
+
 @pytest.mark.parametrize("samples", [1, 5, 16])
 def test_rect_light_stratified_counts(oracle_lib, samples):
     sc = scenes.make_terrain_scene(seed=1, width=48, height=27, quads=32, instances=8, soup_tris=64,

@@ -172,7 +172,8 @@ enum : uint32_t {
     PURPOSE_SURFACE_SHADOW = 1,  // index = light * samples + sample
     PURPOSE_REFLECT = 2,         // index = spp sample
     PURPOSE_HIT_SHADOW = 3,      // index = (sample * max_depth + depth) * nl + light
-    PURPOSE_HIT_REFLECT = 4      // index = sample * max_depth + depth
+    PURPOSE_HIT_REFLECT = 4,     // index = sample * max_depth + depth
+    PURPOSE_ADAPTIVE_SHADOW = 5  // index = light * 128 + k (k < 4 probes, 64 + s refinement)
 };
 HR_DEV Pcg ray_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index) {
     return pcg_init((uint64_t)pixel_seed, ((uint64_t)purpose << 32) | (uint64_t)index);

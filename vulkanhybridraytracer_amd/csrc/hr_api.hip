@@ -183,6 +183,7 @@ hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, con
     f.samples = prm->shadow_samples;
     f.spp = prm->refl_spp;
     f.max_depth = prm->max_depth;
+    f.adaptive = (prm->flags & HR_FLAG_ADAPTIVE) ? 1u : 0u;
     return HR_OK;
 }
 hr_status check_gbuffer(hr_context* ctx, const hr_gbuffer* gb) {

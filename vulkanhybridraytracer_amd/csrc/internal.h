@@ -71,6 +71,7 @@ struct FrameArgs {
     uint32_t width, height;
     uint32_t samples, spp, max_depth;
     uint32_t has_shapes;
+    uint32_t adaptive;  // HR_FLAG_ADAPTIVE
 };
 
 struct GBufView {

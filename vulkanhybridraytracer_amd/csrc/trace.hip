@@ -108,6 +108,8 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
     // current group
     size_t p = 0;
     uint32_t li = 0, seed = 0, s = 0, unocc = 0;
+    uint32_t target = SS;   // rays of the current phase
+    bool refine = false;    // adaptive: the 64-sample phase after disagreeing probes
     v3 P = mk(0.0f, 0.0f, 0.0f), L = P;
     float dist = 0.0f;
     bool isDir = false, in_group = false;
@@ -125,9 +127,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
                 if (s > 0) {  // a ray of this group just finished
                     if (hit) ++c.occluded; else ++unocc;
                 }
-                if (s == SS) {
-                    shadow[li * plane + p] = (float)unocc / (float)SS;
-                    in_group = false;
+                if (s == target) {
+                    if (F.adaptive && !refine && target == 4u && unocc != 0u && unocc != 4u) {
+                        refine = true;  // the probes disagree: a penumbra
+                        target = 64u;
+                        s = 0;
+                        unocc = 0;
+                    } else {
+                        shadow[li * plane + p] = (float)unocc / (float)target;
+                        in_group = false;
+                    }
                 }
             }
             if (!in_group) {
@@ -177,13 +186,21 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
                     seed = tea(y * F.width + x, F.frame);
                     s = 0;
                     unocc = 0;
+                    refine = false;
+                    target = F.adaptive ? (hard_light(F, li) ? 1u : 4u) : SS;
                     in_group = true;
                     break;
                 }
                 if (exhausted) break;
             }
-            Pcg g = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
-            v3 dir = light_sample_dir(F, li, P, L, g, s, SS);
+            v3 dir;
+            if (F.adaptive) {
+                Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (refine ? 64u + s : s));
+                dir = light_sample_dir_strat(F, li, P, L, g, s, target);
+            } else {
+                Pcg g = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
+                dir = light_sample_dir(F, li, P, L, g, s, SS);
+            }
             tmax = isDir ? 1000.0f : dist + 1.0f;
             r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
             ++c.shadow;

@@ -223,6 +223,32 @@ __device__ __forceinline__ v3 light_sample_dir(const FrameArgs& F, uint32_t i, v
         return sampleRect(ld3p(lt.pos_maxdist), F.shapes[i], P, s, S, r1, r2);
     return sampleDisk(lt, P, L, r1, r2);
 }
+// Adaptive-sampling light sample: cell s of the n x ceil(S/n) grid over the
+// light (n = floor(sqrt(S))), for rectangles as sampleRect and for disks on
+// the (radius, angle) square of sampleDisk.
+__device__ __forceinline__ v3 light_sample_dir_strat(const FrameArgs& F, uint32_t i, v3 P, v3 L, Pcg& g,
+                                                     uint32_t s, uint32_t S) {
+    const hr_light& lt = F.lights[i];
+    float r1 = u01(g);
+    float r2 = u01(g);
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    if (!isDir && F.has_shapes && F.shapes[i].type == HR_LIGHT_RECT)
+        return sampleRect(ld3p(lt.pos_maxdist), F.shapes[i], P, s, S, r1, r2);
+    uint32_t n = 1;
+    while ((n + 1) * (n + 1) <= S) ++n;
+    uint32_t m = (S + n - 1) / n;
+    float a = ((float)(s % n) + r1) / (float)n;
+    float b = ((float)(s / n) + r2) / (float)m;
+    return sampleDisk(lt, P, L, a, b);
+}
+// A light whose adaptive sample count is 1: a disk (or direction) of radius 0.
+__device__ __forceinline__ bool hard_light(const FrameArgs& F, uint32_t i) {
+    const hr_light& lt = F.lights[i];
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    bool rect = !isDir && F.has_shapes && F.shapes[i].type == HR_LIGHT_RECT;
+    return !rect && lt.radius == 0.0f;
+}
+
 __device__ __forceinline__ int mat_index(const DevScene& S, float w) {
     int i = (int)w;
     return (i < 0 || i >= (int)S.nmat) ? 0 : i;

@@ -218,8 +218,14 @@ __device__ HitOut shade_hit(const DevScene& S, const FrameArgs& F, v3 rd, const 
 __device__ __forceinline__ size_t slot_of(const WaveArgs& W, uint32_t pixel, uint32_t sample) {
     return (size_t)(pixel - W.p0) * W.spp_max + sample;
 }
+// paths of a reflecting pixel: refl_spp for glossy surfaces (1 for mirrors
+// and glass); adaptive: 1 + floor((refl_spp - 1) * roughness)
 __device__ __forceinline__ uint32_t pixel_spp(const FrameArgs& F, int mode, float roughness) {
-    return (mode == 3 && roughness > 0.0f) ? max(F.spp, 1u) : 1u;
+    if (!(mode == 3 && roughness > 0.0f)) return 1u;
+    uint32_t m = max(F.spp, 1u);
+    if (!F.adaptive) return m;
+    uint32_t k = 1u + (uint32_t)floorf((float)(m - 1u) * roughness);
+    return min(k, m);
 }
 
 // ---- stage 1: direct light and path emission (hybridRaygen.rgen:27-149) ------

@@ -18,11 +18,15 @@ Workloads (sizes in the names):
   hybrid_1080p_1m      512x512-quad value-noise terrain + 256 triangle soups of 2048
                        triangles, 4 rectangular 4x4 area lights, 16 stratified shadow
                        samples per light, GGX glossy reflections at 4 spp, 1 bounce
-  hybrid_1080p_4m      the same generator with a 1024x1024-quad terrain and 1024 soups
+  hybrid_1080p_4m      the same generator with a 1024x1024-quad terrain and 1024 soups,
+                       roughness U[0,1] and adaptive sampling (1-64 shadow rays per
+                       light, 1-64 reflection paths per pixel)
 """
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from . import abi
 
 PRIM_DTYPE = np.dtype([("mesh", "<u4"), ("first_index", "<u4"), ("index_count", "<u4"),
                        ("material", "<i4"), ("model", "<f4", 16)])
@@ -330,7 +334,8 @@ def make_small(seed=0, width=640, height=360, spheres=64, frame=0.0):
 
 
 def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256, soup_tris=2048,
-                       extent=200.0, samples=16, spp=4, max_depth=1, frame=0.0, name=None):
+                       extent=200.0, samples=16, spp=4, max_depth=1, frame=0.0, name=None, rough_max=0.6,
+                       adaptive=False):
     """hybrid_1080p_1m (quads=512, instances=256) / hybrid_1080p_4m (1024, 1024):
     value-noise terrain (6 octaves, amplitude 20) with a glossy material
     (shading model 3, metallic 1, roughness U[0,0.6]); triangle-soup instances
@@ -340,14 +345,14 @@ def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256
     rng = np.random.default_rng(seed)
     tV, tI, height_at = terrain_mesh(quads, extent, 20.0, rng)
     meshes = [(tV, tI)]
-    mats = [material(0, rng.uniform(0.3, 0.9, size=3), 3, 1.0, rng.uniform(0.0, 0.6))]
+    mats = [material(0, rng.uniform(0.3, 0.9, size=3), 3, 1.0, rng.uniform(0.0, rough_max))]
     rows = [(0, 0, len(tI), 0, np.eye(4))]
     half = extent * 0.45
     for i in range(instances):
         meshes.append(soup_mesh(soup_tris, rng))
         glossy = rng.uniform() < 0.5
         mats.append(material(1 + i, rng.uniform(0, 1, size=3), 3 if glossy else 0,
-                             1.0 if glossy else 0.0, rng.uniform(0.0, 0.6)))
+                             1.0 if glossy else 0.0, rng.uniform(0.0, rough_max)))
         s = rng.uniform(1.0, 5.0)
         x, z = rng.uniform(-half, half, size=2)
         y = height_at(x, z) + 0.5 * s + 0.5
@@ -365,7 +370,8 @@ def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256
     cam = rt_camera((0, 0.3 * extent, 0.62 * extent), (0, 0, -0.05 * extent), width, height, frame=frame)
     nm = name or ("hybrid_%dx%d_%dtri" % (width, height, quads * quads * 2 + instances * soup_tris))
     return Scene(nm, meshes, _prims(rows), np.stack(mats), np.stack(lights), np.stack(shapes), cam, sky,
-                 irr, width, height, dict(shadow_samples=samples, refl_spp=spp, max_depth=max_depth))
+                 irr, width, height, dict(shadow_samples=samples, refl_spp=spp, max_depth=max_depth,
+                                          flags=abi.HR_FLAG_ADAPTIVE if adaptive else 0))
 
 
 def make_1080p_1m(seed=0, **kw):
@@ -373,6 +379,13 @@ def make_1080p_1m(seed=0, **kw):
 
 
 def make_1080p_4m(seed=0, **kw):
+    """The 4.2M-triangle scene with adaptive sampling: shadows 1 / 4 / 64 rays
+    per light by penumbra probes, reflections 1 + floor(63 * roughness) paths,
+    roughness U[0,1]."""
+    kw.setdefault("samples", 64)
+    kw.setdefault("spp", 64)
+    kw.setdefault("rough_max", 1.0)
+    kw.setdefault("adaptive", True)
     return make_terrain_scene(seed, quads=1024, instances=1024, extent=400.0, name="hybrid_1080p_4m", **kw)
 
 
