@@ -283,6 +283,7 @@ hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
     w.p1 = 0;
     w.spp_max = spp_max;
     w.slab_pixels = (uint32_t)slab;
+    w.blocks = ctx->resident_blocks;
     return HR_OK;
 }
 

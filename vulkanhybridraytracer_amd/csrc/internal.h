@@ -115,8 +115,9 @@ struct WaveArgs {
     uint32_t* pending;
     float4* pathcol;
     float4* pix;
-    uint32_t* counts;  // [0], [1] path queues, [2] shadow rays, [3] pending
+    uint32_t* counts;  // [0], [1] path queues, [2] shadow rays, [3] pending, [4], [5] take counters
     uint32_t p0, p1, spp_max, slab_pixels;
+    uint32_t blocks;   // persistent grid of the trace stages
 };
 size_t wave_bytes_per_slot(uint32_t nl);
 hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, const float* shadow, float4* lit,

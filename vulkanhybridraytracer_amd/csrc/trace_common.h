@@ -270,5 +270,133 @@ __device__ __forceinline__ void flush_counters(const Cnt& c, unsigned long long*
     }
 }
 
+// Wave-aggregated take of one index per calling lane from *next: one atomic
+// for all lanes of the wave that reach this call together.
+__device__ __forceinline__ uint32_t wave_take(uint32_t* next) {
+    const uint32_t lane = threadIdx.x & 63u;
+    unsigned long long need = __ballot(1);
+    uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+    uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(next, (uint32_t)__popcll(need));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    return base + rank;
+}
+
+// Persistent consumer of a ray queue of length n. Each lane traces ray after
+// ray; when its ray finishes it takes the next index from *next, so lanes do
+// not wait for the slowest ray of the wave. Per ray the walk is the shared
+// near-first stack traversal (same order and counts as traverse()).
+//   get(i, o, d, tmin, tmax) -> false for an empty queue entry (skipped)
+//   put(i, hit, HitD)         result of entry i
+template <bool ANY, bool COUNT, class GetRay, class PutHit>
+__device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint32_t* lds, Cnt& c, GetRay get,
+                             PutHit put) {
+    constexpr uint32_t DONE = 0xFFFFFFFFu;  // no leaf code reaches it (fewer than 2^27 triangles)
+    uint32_t cur = DONE, idx = 0;
+    bool have = false;
+    RayP r;
+    float tmin = 0.0f, tbest = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t best = 0xFFFFFFFFu;
+    int sp = 0;
+    uint32_t spill[STACK_SPILL];
+    for (;;) {
+        if (cur == DONE) {
+            if (have) {
+                HitD h;
+                h.t = tbest;
+                h.u = bu;
+                h.v = bv;
+                h.tri = best;
+                put(idx, best != 0xFFFFFFFFu, h);
+                have = false;
+            }
+            bool got = false;
+            for (;;) {
+                idx = wave_take(next);
+                if (idx >= n) break;
+                v3 o, d;
+                float tmax;
+                if (get(idx, o, d, tmin, tmax)) {
+                    r = prep_ray(o, d);
+                    tbest = tmax;
+                    got = true;
+                    break;
+                }
+            }
+            if (!got) break;
+            have = true;
+            best = 0xFFFFFFFFu;
+            sp = 0;
+            cur = S.node_count ? 0u : DONE;
+            continue;
+        }
+        while (cur < LEAF_BIT) {
+            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 a = np[0], b = np[1], cc = np[2], d = np[3];
+            if (COUNT) ++c.nodes;
+            float tn0, tn1;
+            bool h0 = box_hit(a.x, a.y, a.z, b.x, b.y, b.z, r, tmin, tbest, tn0);
+            bool h1 = box_hit(cc.x, cc.y, cc.z, d.x, d.y, d.z, r, tmin, tbest, tn1);
+            uint32_t c0 = __float_as_uint(a.w), c1 = __float_as_uint(b.w);
+            if (h0 && h1) {
+                uint32_t nearc = c0, farc = c1;
+                if (tn1 < tn0) {
+                    nearc = c1;
+                    farc = c0;
+                }
+                if (sp < (int)STACK_LDS) lds[sp * BLOCK] = farc;
+                else if (sp < (int)MAX_DEPTH) spill[sp - STACK_LDS] = farc;
+                else c.overflow = 1;
+                sp = min(sp + 1, (int)MAX_DEPTH);
+                cur = nearc;
+            } else if (h0) {
+                cur = c0;
+            } else if (h1) {
+                cur = c1;
+            } else if (sp == 0) {
+                cur = DONE;
+            } else {
+                --sp;
+                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+            }
+        }
+        if (cur != DONE) {
+            uint32_t first = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            bool stop = false;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                if (COUNT) ++c.tris;
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tbest, t, b1, b2)) {
+                    uint32_t id = __float_as_uint(v0.w);
+                    if (ANY) {
+                        tbest = t;
+                        best = id;
+                        bu = b1;
+                        bv = b2;
+                        stop = true;
+                        break;
+                    }
+                    if (t < tbest || id < best) {
+                        tbest = t;
+                        best = id;
+                        bu = b1;
+                        bv = b2;
+                    }
+                }
+            }
+            if (stop || sp == 0) {
+                cur = DONE;
+            } else {
+                --sp;
+                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+            }
+        }
+    }
+}
+
 }  // namespace tc
 }  // namespace hr

@@ -368,31 +368,32 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
 }
 
 // ---- bounce stages -------------------------------------------------------------
+// closest hit of every queued path (persistent, dynamic fetch)
 template <bool COUNT>
 __global__ __launch_bounds__(WF_BLOCK) void k_wf_trace(DevScene S, const float4* paths, const uint32_t* count,
-                                                       float4* hits, unsigned long long* counters) {
+                                                       float4* hits, uint32_t* next, unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * WF_BLOCK];
     uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
     uint32_t n = *count;
-    uint32_t stride = gridDim.x * WF_BLOCK;
-    for (uint32_t base = blockIdx.x * WF_BLOCK; base < n; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = paths[3 * (size_t)i], b = paths[3 * (size_t)i + 1];
-            float4 h4 = make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
-            if (__float_as_uint(a.w) != 0xFFFFFFFFu) {
-                v3 d = ld3(b);
-                v3 ro = add(ld3(a), scl(d, 1e-2f));
-                RayP r = prep_ray(ro, d);
-                HitD h;
-                ++c.refl;
-                if (traverse<false, COUNT>(S, r, 0.001f, 1000.0f, lds, h, c))
-                    h4 = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
-            }
-            hits[i] = h4;
+    auto get = [&](uint32_t i, v3& o, v3& d, float& tmin, float& tmax) {
+        float4 a = paths[3 * (size_t)i], b = paths[3 * (size_t)i + 1];
+        if (__float_as_uint(a.w) == 0xFFFFFFFFu) {  // reserved but empty slot
+            hits[i] = make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
+            return false;
         }
-    }
+        d = ld3(b);
+        o = add(ld3(a), scl(d, 1e-2f));
+        tmin = 0.001f;
+        tmax = 1000.0f;
+        ++c.refl;
+        return true;
+    };
+    auto put = [&](uint32_t i, bool hit, const HitD& h) {
+        hits[i] = hit ? make_float4(h.t, h.u, h.v, __uint_as_float(h.tri))
+                      : make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
+    };
+    consume_rays<false, COUNT>(S, n, next, lds, c, get, put);
     flush_counters(c, counters, COUNT);
 }
 
@@ -480,24 +481,32 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_shade(DevScene S, FrameArgs F, 
     }
 }
 
+// any hit of every hit shadow ray; unoccluded rays set their light's bit
 template <bool COUNT>
-__global__ __launch_bounds__(WF_BLOCK) void k_wf_shadow(DevScene S, WaveArgs W, unsigned long long* counters) {
+__global__ __launch_bounds__(WF_BLOCK) void k_wf_shadow(DevScene S, WaveArgs W, uint32_t* next,
+                                                        unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * WF_BLOCK];
     uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
     uint32_t n = W.counts[2];
-    uint32_t stride = gridDim.x * WF_BLOCK;
-    for (uint32_t base = blockIdx.x * WF_BLOCK; base < n; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        if (i < n) {
-            float4 a = W.srays[2 * (size_t)i], b = W.srays[2 * (size_t)i + 1];
-            uint32_t id = __float_as_uint(b.w);
-            ++c.hitshadow;
-            bool occ = occluded<COUNT>(S, ld3(a), ld3(b), 0.001f, a.w, lds, c);
-            if (occ) ++c.occluded;
-            else atomicOr(&W.vis[id >> 4], 1u << (id & 15u));
+    auto get = [&](uint32_t i, v3& o, v3& d, float& tmin, float& tmax) {
+        float4 a = W.srays[2 * (size_t)i], b = W.srays[2 * (size_t)i + 1];
+        o = ld3(a);
+        d = ld3(b);
+        tmin = 0.001f;
+        tmax = a.w;
+        ++c.hitshadow;
+        return true;
+    };
+    auto put = [&](uint32_t i, bool hit, const HitD&) {
+        if (hit) {
+            ++c.occluded;
+        } else {
+            uint32_t id = __float_as_uint(W.srays[2 * (size_t)i + 1].w);
+            atomicOr(&W.vis[id >> 4], 1u << (id & 15u));
         }
-    }
+    };
+    consume_rays<true, COUNT>(S, n, next, lds, c, get, put);
     flush_counters(c, counters, COUNT);
 }
 
@@ -543,7 +552,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_resolve(FrameArgs F, GBufView G
 
 __global__ void k_wf_reset(uint32_t* counts, uint32_t mask) {
     uint32_t i = threadIdx.x;
-    if (i < 4 && ((mask >> i) & 1u)) counts[i] = 0u;
+    if (i < 16 && ((mask >> i) & 1u)) counts[i] = 0u;
 }
 
 constexpr uint32_t STAGE_BLOCKS = 2048;  // 8 workgroups per CU, each striding over the queue
@@ -570,12 +579,13 @@ hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, cons
             float4* nxt = (d & 1) ? w.paths0 : w.paths1;
             uint32_t* ccur = &w.counts[d & 1];
             uint32_t* cnxt = &w.counts[(d + 1) & 1];
-            k_wf_reset<<<1, 64, 0, st>>>(w.counts, (1u << ((d + 1) & 1)) | 0xCu);
-            if (count) k_wf_trace<true><<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, counters);
-            else k_wf_trace<false><<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, counters);
+            // counts[4] / counts[5]: take counters of the two trace stages
+            k_wf_reset<<<1, 64, 0, st>>>(w.counts, (1u << ((d + 1) & 1)) | 0x3Cu);
+            if (count) k_wf_trace<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, &w.counts[4], counters);
+            else k_wf_trace<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, &w.counts[4], counters);
             k_wf_shade<<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, f, w, cur, nxt, ccur, cnxt, d);
-            if (count) k_wf_shadow<true><<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, w, counters);
-            else k_wf_shadow<false><<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, w, counters);
+            if (count) k_wf_shadow<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, &w.counts[5], counters);
+            else k_wf_shadow<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, &w.counts[5], counters);
             k_wf_finish<<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, f, w, cur, d);
         }
         k_wf_resolve<<<nb, WF_BLOCK, 0, st>>>(f, gb, w, lit, refl);
