@@ -178,7 +178,7 @@ typedef struct hr_gbuffer {
  *   (1 in the reference).
  * max_depth: bounce limit (MAX_RECURSION = 8, helpers.glsl:8). */
 typedef struct hr_trace_params {
-    uint32_t shadow_samples;  /* unused with HR_FLAG_ADAPTIVE */
+    uint32_t shadow_samples;  /* with HR_FLAG_ADAPTIVE: the penumbra sample count (<= 64) */
     uint32_t refl_spp;
     uint32_t max_depth;
     uint32_t flags;
@@ -225,8 +225,9 @@ typedef struct hr_stats {
 /* Adaptive sampling (no counterpart in the reference, whose sample count is a
  * uniform, renderer.cpp:656):
  *  - shadows: a disk light of radius 0 takes 1 ray; other lights take 4
- *    stratified probe rays and, when the probes disagree (a penumbra), 64
- *    stratified rays that replace them; shadow_samples is then unused;
+ *    stratified probe rays and, when the probes disagree (a penumbra) and
+ *    shadow_samples > 4, shadow_samples stratified rays (at most 64) that
+ *    replace them;
  *  - reflections: a glossy pixel (shading model 3, roughness r > 0) traces
  *    1 + floor((refl_spp - 1) * r) paths. */
 #define HR_FLAG_ADAPTIVE 4u
@@ -302,6 +303,54 @@ hr_status hr_shade_hybrid(hr_scene* scene, const hr_gbuffer* gb,
  * broken by the smaller triangle id. */
 hr_status hr_trace_rays(hr_scene* scene, const hr_ray* rays, uint32_t n, int any_hit,
                         hr_hit* hits, hr_stats* stats);
+
+/* ---- batches of frames (no counterpart in the reference, which renders one
+ * scene per frame) ----------------------------------------------------------
+ * A frame is one scene with its G-buffer, lights, camera, parameters and
+ * device output buffers, as hr_trace_shadows / hr_shade_hybrid take them. All
+ * scenes must belong to the same context. */
+typedef struct hr_frame_desc {
+    hr_scene* scene;
+    hr_gbuffer gbuffer;
+    const hr_light* lights;
+    const hr_light_shape* shapes; /* may be NULL */
+    uint32_t light_count;
+    hr_camera camera;
+    hr_trace_params params;
+    float* shadow;  /* device float[light_count][H][W] */
+    float* lit;     /* device float4[H][W] */
+    float* refl;    /* device float4[H][W] */
+} hr_frame_desc;
+
+/* The shadow pass of every frame in one persistent kernel; work is taken
+ * from one device-side counter shared by all frames. */
+hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_stats* stats);
+/* The hybrid shading pass of every frame (after hr_trace_shadows_batch). */
+hr_status hr_shade_hybrid_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_stats* stats);
+
+/* A frame of a batch (shadows, then shading) captured once into a hipGraph
+ * and replayed by hr_graph_launch. The graph owns its device copies of the
+ * frames' parameters and its own workspace; the frames' scenes and buffers
+ * must outlive it. */
+typedef struct hr_graph hr_graph;
+hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_graph** out);
+hr_status hr_graph_launch(hr_graph* graph, hr_stats* stats);
+void hr_graph_destroy(hr_graph* graph);
+
+/* Launch shapes, set per context (hr_context_set_tuning) and chosen by the
+ * autotuner of the Python package:
+ *  blocks_per_cu   persistent trace grid, 256-thread workgroups per CU (1..16)
+ *  tile_w          shadow work tile width in pixels (1..64, power of two;
+ *                  height 64 / tile_w): one wave's 64 groups
+ *  wave_budget_mb  reflection-queue workspace per pixel slab, MiB (>= 64) */
+typedef struct hr_tuning {
+    uint32_t blocks_per_cu;
+    uint32_t tile_w;
+    uint32_t wave_budget_mb;
+    uint32_t _reserved;
+} hr_tuning;
+hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
+hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
 
 #ifdef __cplusplus
 }

@@ -1320,7 +1320,7 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
                 float NdotL = clamp01(dot(N, L));
                 float factor = 0.0f;
                 if (NdotL > 0.0f && f.adaptive && (isDir || dist < maxd)) {
-                    /* 1 ray for hard lights, else 4 probes; 64 rays when they disagree */
+                    /* 1 ray for hard lights, else 4 probes; shadow_samples rays when they disagree */
                     uint32_t target = hard_light(f, i) ? 1u : 4u;
                     bool refine = false;
                     for (;;) {
@@ -1333,9 +1333,9 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
                             bool sh = occluded(*s, add(P, scl(dir, 1e-1f)), dir, 0.001f, tmax, c);
                             if (sh) ++c.occluded; else ++unocc;
                         }
-                        if (!refine && target == 4u && unocc != 0u && unocc != 4u) {
+                        if (!refine && target == 4u && unocc != 0u && unocc != 4u && S > 4u) {
                             refine = true;
-                            target = 64u;
+                            target = S;
                             continue;
                         }
                         factor = (float)unocc / (float)target;

@@ -92,6 +92,17 @@ class hr_build_params(C.Structure):
     _fields_ = [("leaf_max", C.c_uint32), ("_reserved", C.c_uint32 * 3)]
 
 
+class hr_frame_desc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("gbuffer", hr_gbuffer), ("lights", C.c_void_p), ("shapes", C.c_void_p),
+                ("light_count", C.c_uint32), ("camera", hr_camera), ("params", hr_trace_params),
+                ("shadow", C.c_void_p), ("lit", C.c_void_p), ("refl", C.c_void_p)]
+
+
+class hr_tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("tile_w", C.c_uint32), ("wave_budget_mb", C.c_uint32),
+                ("_reserved", C.c_uint32)]
+
+
 for _s, _n in ((hr_vertex, 44), (hr_material, 48), (hr_light, 48), (hr_light_shape, 32),
                (hr_camera, 144), (hr_primitive, 80), (hr_ray, 32), (hr_hit, 16)):
     assert C.sizeof(_s) == _n, (_s.__name__, C.sizeof(_s))

@@ -20,9 +20,13 @@ struct hr_context {
     hipEvent_t ev[2] = {nullptr, nullptr};
     unsigned long long* d_counters = nullptr;
     uint32_t* d_next_group = nullptr;  // work counter of the persistent shadow kernel
-    uint32_t resident_blocks = 2048;    // 256-thread workgroups resident at once
+    uint32_t cu_count = 256;
+    uint32_t resident_blocks = 2048;    // persistent grid: cu_count x tuning.blocks_per_cu
+    hr_tuning tuning{8u, 8u, 3072u, 0u};
     void* work = nullptr;  // grow-only workspace of the reflection pipeline
     size_t work_bytes = 0;
+    ShadowJob* d_jobs = nullptr;  // grow-only device copy of the shadow jobs of a call
+    uint32_t jobs_cap = 0;
     std::string err;
 };
 
@@ -166,6 +170,8 @@ hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, con
     if (!prm) return fail(ctx, HR_E_INVALID, "trace params are NULL");
     if (prm->shadow_samples == 0 || prm->shadow_samples > 4096)
         return fail(ctx, HR_E_INVALID, "shadow_samples must be in 1..4096");
+    if ((prm->flags & HR_FLAG_ADAPTIVE) && prm->shadow_samples > 64)
+        return fail(ctx, HR_E_INVALID, "adaptive sampling takes at most 64 penumbra samples");
     if (prm->refl_spp > 4096 || prm->max_depth > 64)
         return fail(ctx, HR_E_INVALID, "refl_spp must be <= 4096 and max_depth <= 64");
     for (uint32_t i = 0; i < nl; ++i) {
@@ -235,7 +241,6 @@ hr_status frame_end(hr_context* ctx, hr_stats* stats, float* ms_slot) {
 
 // Workspace budget of one slab of the reflection pipeline; larger frames are
 // processed in slabs of pixels.
-constexpr size_t WORK_BUDGET = size_t(3) << 30;
 
 hr_status ensure_work(hr_context* ctx, size_t bytes) {
     if (bytes <= ctx->work_bytes) return HR_OK;
@@ -250,15 +255,18 @@ hr_status ensure_work(hr_context* ctx, size_t bytes) {
     return HR_OK;
 }
 
-hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
+// Layout of the reflection workspace for frame f: returns its size and, with
+// a base address, fills w. Frames larger than the budget run in pixel slabs.
+size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char* b) {
     uint32_t spp_max = f.spp > 1 ? f.spp : 1;
     size_t per_slot = wave_bytes_per_slot(f.nl);
     size_t npix = (size_t)f.width * f.height;
-    size_t slab = WORK_BUDGET / (per_slot * spp_max + 32);
+    size_t budget = (size_t)ctx->tuning.wave_budget_mb << 20;
+    size_t slab = budget / (per_slot * spp_max + 32);
+    if (slab < 256) slab = 256;
     slab = slab < npix ? slab : npix;
     slab = (slab + 255) / 256 * 256;
     size_t slots = slab * spp_max;
-    if (slots >= (size_t(1) << 28)) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         size_t o = off;
@@ -268,23 +276,94 @@ hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
     size_t o_p0 = carve(slots * 48), o_p1 = carve(slots * 48), o_hits = carve(slots * 16);
     size_t o_sr = carve(slots * f.nl * 32), o_vis = carve(slots * 4), o_pend = carve(slots * 4);
     size_t o_col = carve(slots * 16), o_pix = carve(slab * 32), o_cnt = carve(64);
-    HR_TRY(ensure_work(ctx, off));
-    char* b = static_cast<char*>(ctx->work);
-    w.paths0 = reinterpret_cast<float4*>(b + o_p0);
-    w.paths1 = reinterpret_cast<float4*>(b + o_p1);
-    w.hits = reinterpret_cast<float4*>(b + o_hits);
-    w.srays = reinterpret_cast<float4*>(b + o_sr);
-    w.vis = reinterpret_cast<uint32_t*>(b + o_vis);
-    w.pending = reinterpret_cast<uint32_t*>(b + o_pend);
-    w.pathcol = reinterpret_cast<float4*>(b + o_col);
-    w.pix = reinterpret_cast<float4*>(b + o_pix);
-    w.counts = reinterpret_cast<uint32_t*>(b + o_cnt);
-    w.p0 = 0;
-    w.p1 = 0;
-    w.spp_max = spp_max;
-    w.slab_pixels = (uint32_t)slab;
-    w.blocks = ctx->resident_blocks;
+    if (b) {
+        w.paths0 = reinterpret_cast<float4*>(b + o_p0);
+        w.paths1 = reinterpret_cast<float4*>(b + o_p1);
+        w.hits = reinterpret_cast<float4*>(b + o_hits);
+        w.srays = reinterpret_cast<float4*>(b + o_sr);
+        w.vis = reinterpret_cast<uint32_t*>(b + o_vis);
+        w.pending = reinterpret_cast<uint32_t*>(b + o_pend);
+        w.pathcol = reinterpret_cast<float4*>(b + o_col);
+        w.pix = reinterpret_cast<float4*>(b + o_pix);
+        w.counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+        w.p0 = 0;
+        w.p1 = 0;
+        w.spp_max = spp_max;
+        w.slab_pixels = (uint32_t)slab;
+        w.blocks = ctx->resident_blocks;
+    }
+    return slots >= (size_t(1) << 28) ? 0 : off;
+}
+
+hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
+    size_t bytes = wave_layout(ctx, f, w, nullptr);
+    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+    HR_TRY(ensure_work(ctx, bytes));
+    wave_layout(ctx, f, w, static_cast<char*>(ctx->work));
     return HR_OK;
+}
+
+hr_status ensure_jobs(hr_context* ctx, uint32_t n) {
+    if (n <= ctx->jobs_cap) return HR_OK;
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_jobs) {
+        HR_HIP(ctx, hipFree(ctx->d_jobs));
+        ctx->d_jobs = nullptr;
+        ctx->jobs_cap = 0;
+    }
+    HR_HIP(ctx, hipMalloc(&ctx->d_jobs, n * sizeof(ShadowJob)));
+    ctx->jobs_cap = n;
+    return HR_OK;
+}
+
+// One frame of a batch, validated and converted to kernel arguments.
+struct FramePlan {
+    hr_scene* scene;
+    FrameArgs f;
+    GBufView g;
+    float* shadow;
+    float* lit;
+    float* refl;
+    bool count;
+};
+
+hr_status plan_frame(hr_context* ctx, const hr_frame_desc& d, FramePlan& fp, bool need_shading) {
+    if (!d.scene) return fail(ctx, HR_E_INVALID, "frame without a scene");
+    if (d.scene->ctx != ctx) return fail(ctx, HR_E_INVALID, "scene belongs to another context");
+    HR_TRY(check_gbuffer(ctx, &d.gbuffer));
+    if (!d.shadow) return fail(ctx, HR_E_INVALID, "shadow output is NULL");
+    if (need_shading && (!d.lit || !d.refl)) return fail(ctx, HR_E_INVALID, "lit/refl output is NULL");
+    HR_TRY(fill_frame(ctx, fp.f, &d.camera, d.gbuffer.width, d.gbuffer.height));
+    HR_TRY(fill_lights(ctx, fp.f, d.lights, d.shapes, d.light_count, &d.params));
+    if (need_shading && d.params.refl_spp > 64) return fail(ctx, HR_E_UNSUPPORTED, "refl_spp above 64");
+    fp.scene = d.scene;
+    fp.g = gview(&d.gbuffer);
+    fp.shadow = d.shadow;
+    fp.lit = d.lit;
+    fp.refl = d.refl;
+    fp.count = (d.params.flags & HR_FLAG_COUNT_VISITS) != 0;
+    return HR_OK;
+}
+
+// Shadow jobs of a batch (host side); returns the total group count.
+uint32_t make_jobs(const hr_context* ctx, const std::vector<FramePlan>& plans, std::vector<ShadowJob>& jobs) {
+    uint32_t tw = ctx->tuning.tile_w, th = 64u / tw;
+    uint32_t base = 0;
+    jobs.resize(plans.size());
+    for (size_t i = 0; i < plans.size(); ++i) {
+        ShadowJob& j = jobs[i];
+        memset(&j, 0, sizeof(j));
+        j.S = dev_scene(plans[i].scene);
+        j.F = plans[i].f;
+        j.G = plans[i].g;
+        j.shadow = plans[i].shadow;
+        j.group_base = base;
+        j.tile_w = tw;
+        j.tiles_x = (plans[i].f.width + tw - 1u) / tw;
+        (void)th;
+        base += shadow_groups(plans[i].f, tw);
+    }
+    return base;
 }
 
 }  // namespace
@@ -310,7 +389,8 @@ hr_status hr_context_create(int device, hr_context** out) {
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-            ctx->resident_blocks = (uint32_t)prop.multiProcessorCount * 8u;  // 8 x 4 waves per CU
+            ctx->cu_count = (uint32_t)prop.multiProcessorCount;
+        ctx->resident_blocks = ctx->cu_count * ctx->tuning.blocks_per_cu;
     }
     if (e != hipSuccess) {
         hr_context_destroy(ctx);
@@ -326,6 +406,7 @@ void hr_context_destroy(hr_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_next_group) (void)hipFree(ctx->d_next_group);
+    if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
     if (ctx->work) (void)hipFree(ctx->work);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -709,9 +790,15 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
     HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
     bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    std::vector<FramePlan> plans(1);
+    plans[0] = FramePlan{s, f, gview(gb), shadow, nullptr, nullptr, count};
+    std::vector<ShadowJob> jobs;
+    uint32_t ngroups = make_jobs(ctx, plans, jobs);
+    HR_TRY(ensure_jobs(ctx, 1));
     HR_TRY(frame_begin(ctx, stats != nullptr));
-    HR_HIP(ctx, launch_shadows(dev_scene(s), f, gview(gb), shadow, ctx->d_counters, count, ctx->stream,
-                               ctx->d_next_group, ctx->resident_blocks));
+    HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
+    HR_HIP(ctx, launch_shadows(ctx->d_jobs, 1, ngroups, ctx->d_counters, count, ctx->stream, ctx->d_next_group,
+                               ctx->resident_blocks));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -743,6 +830,180 @@ hr_status hr_trace_rays(hr_scene* s, const hr_ray* rays, uint32_t n, int any_hit
     HR_HIP(ctx, launch_trace_rays(dev_scene(s), rays, n, any_hit, hits, ctx->d_counters, stats != nullptr,
                                   ctx->stream));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
+}
+
+hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* t) {
+    if (!ctx) return HR_E_INVALID;
+    if (!t) return fail(ctx, HR_E_INVALID, "tuning is NULL");
+    bool pow2 = t->tile_w && (t->tile_w & (t->tile_w - 1)) == 0;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || !pow2 || t->tile_w > 64 || t->wave_budget_mb < 64)
+        return fail(ctx, HR_E_INVALID, "tuning out of range");
+    ctx->tuning = *t;
+    ctx->tuning._reserved = 0;
+    ctx->resident_blocks = ctx->cu_count * t->blocks_per_cu;
+    return HR_OK;
+}
+
+hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* t) {
+    if (!ctx || !t) return HR_E_INVALID;
+    *t = ctx->tuning;
+    return HR_OK;
+}
+
+}  // extern "C"
+
+// ---- batches and graphs --------------------------------------------------------
+namespace {
+
+hr_status plan_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, bool shading,
+                     std::vector<FramePlan>& plans) {
+    if (!frames || n == 0) return fail(ctx, HR_E_INVALID, "empty batch");
+    plans.resize(n);
+    for (uint32_t i = 0; i < n; ++i) HR_TRY(plan_frame(ctx, frames[i], plans[i], shading));
+    return HR_OK;
+}
+
+// Kernels of a batch's shadow pass (device jobs already uploaded).
+hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, uint32_t ngroups, bool count,
+                          uint32_t* next_group) {
+    HR_HIP(ctx, launch_shadows(d_jobs, n, ngroups, ctx->d_counters, count, ctx->stream, next_group,
+                               ctx->resident_blocks));
+    return HR_OK;
+}
+
+// Kernels of a batch's shading pass, frames one after another on a shared
+// workspace at base (laid out per frame).
+hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base) {
+    for (const FramePlan& fp : plans) {
+        WaveArgs w;
+        wave_layout(ctx, fp.f, w, base);
+        HR_HIP(ctx, launch_shade(dev_scene(fp.scene), fp.f, fp.g, fp.shadow, reinterpret_cast<float4*>(fp.lit),
+                                 reinterpret_cast<float4*>(fp.refl), ctx->d_counters, fp.count, ctx->stream, w));
+    }
+    return HR_OK;
+}
+
+size_t batch_work_bytes(hr_context* ctx, const std::vector<FramePlan>& plans) {
+    size_t m = 0;
+    for (const FramePlan& fp : plans) {
+        WaveArgs w;
+        size_t b = wave_layout(ctx, fp.f, w, nullptr);
+        if (b == 0) return 0;
+        m = b > m ? b : m;
+    }
+    return m;
+}
+
+}  // namespace
+
+struct hr_graph {
+    hr_context* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ShadowJob* d_jobs = nullptr;
+    uint32_t* d_next = nullptr;
+    void* work = nullptr;
+};
+
+extern "C" {
+
+hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_stats* stats) {
+    if (!ctx) return HR_E_INVALID;
+    std::vector<FramePlan> plans;
+    HR_TRY(plan_batch(ctx, frames, n, false, plans));
+    std::vector<ShadowJob> jobs;
+    uint32_t ngroups = make_jobs(ctx, plans, jobs);
+    bool count = false;
+    for (auto& fp : plans) count = count || fp.count;
+    HR_TRY(ensure_jobs(ctx, n));
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
+    HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, n, ngroups, count, ctx->d_next_group));
+    return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
+}
+
+hr_status hr_shade_hybrid_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_stats* stats) {
+    if (!ctx) return HR_E_INVALID;
+    std::vector<FramePlan> plans;
+    HR_TRY(plan_batch(ctx, frames, n, true, plans));
+    size_t bytes = batch_work_bytes(ctx, plans);
+    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+    HR_TRY(ensure_work(ctx, bytes));
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_TRY(enqueue_shading(ctx, plans, static_cast<char*>(ctx->work)));
+    return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
+}
+
+void hr_graph_destroy(hr_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_jobs) (void)hipFree(g->d_jobs);
+    if (g->d_next) (void)hipFree(g->d_next);
+    if (g->work) (void)hipFree(g->work);
+    delete g;
+}
+
+hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_graph** out) {
+    if (!ctx) return HR_E_INVALID;
+    if (!out) return fail(ctx, HR_E_INVALID, "out is NULL");
+    *out = nullptr;
+    std::vector<FramePlan> plans;
+    HR_TRY(plan_batch(ctx, frames, n, true, plans));
+    std::vector<ShadowJob> jobs;
+    uint32_t ngroups = make_jobs(ctx, plans, jobs);
+    size_t bytes = batch_work_bytes(ctx, plans);
+    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+    bool count = false;
+    for (auto& fp : plans) count = count || fp.count;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hr_graph* g = new hr_graph();
+    g->ctx = ctx;
+    hipError_t e = hipMalloc(&g->d_jobs, n * sizeof(ShadowJob));
+    if (e == hipSuccess) e = hipMalloc(&g->d_next, 64);
+    if (e == hipSuccess) e = hipMalloc(&g->work, bytes);
+    if (e == hipSuccess) e = hipMemcpy(g->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hr_graph_destroy(g);
+        return hip_fail(ctx, e, "graph buffers");
+    }
+    // capture: counters cleared by a kernel (no memset nodes), shadows, shading
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hr_graph_destroy(g);
+        return hip_fail(ctx, e, "hipStreamBeginCapture");
+    }
+    hr_status st = HR_OK;
+    hipError_t le = launch_zero_u64(ctx->d_counters, C_COUNT, ctx->stream);
+    if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
+    if (st == HR_OK) st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, count, g->d_next);
+    if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work));
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    g->graph = graph;
+    if (st != HR_OK || e != hipSuccess) {
+        hr_graph_destroy(g);
+        return st != HR_OK ? st : hip_fail(ctx, e, "hipStreamEndCapture");
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hr_graph_destroy(g);
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    *out = g;
+    return HR_OK;
+}
+
+hr_status hr_graph_launch(hr_graph* g, hr_stats* stats) {
+    if (!g) return HR_E_INVALID;
+    hr_context* ctx = g->ctx;
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    if (stats) HR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    HR_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 
 }  // extern "C"

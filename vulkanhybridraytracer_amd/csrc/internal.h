@@ -98,11 +98,23 @@ enum : int {
 // ---- launchers (trace.hip) ---------------------------------------------------
 hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, unsigned long long* counters,
                           bool count, hipStream_t st);
-// next_group: one device uint32 (reset by the launch); resident_blocks: the
-// persistent grid size (workgroups that fit on the device at once)
-hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
-                          unsigned long long* counters, bool count, hipStream_t st, uint32_t* next_group,
-                          uint32_t resident_blocks);
+// One frame of a shadow launch, in device memory: the launch's groups
+// (pixel, light) are numbered frame after frame from group_base.
+struct ShadowJob {
+    DevScene S;
+    FrameArgs F;
+    GBufView G;
+    float* shadow;
+    uint32_t group_base;
+    uint32_t tile_w;   // pixel tile of one wave: tile_w x (64 / tile_w)
+    uint32_t tiles_x;  // tiles per row
+    uint32_t _pad;
+};
+uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w);
+// jobs_dev: njobs ShadowJob in device memory; next_group: one device uint32
+// (reset by the launch); resident_blocks: the persistent grid size
+hipError_t launch_shadows(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, unsigned long long* counters,
+                          bool count, hipStream_t st, uint32_t* next_group, uint32_t resident_blocks);
 // Workspace of the wavefront reflection pipeline for one slab of pixels
 // [p0, p1): path queues (ping-pong), hits, hit shadow rays, visibility masks,
 // pending list, per (pixel, sample) results and per-pixel direct / ambient.

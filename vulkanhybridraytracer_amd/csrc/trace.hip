@@ -81,19 +81,21 @@ __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBuf
     flush_counters(c, counters, COUNT);
 }
 
-// Shadow pass, persistent waves with dynamic fetch. The work unit is a
-// group = (pixel, light): its `samples` rays are traced by one lane in
-// sample order and its shadow factor is written once by that lane. Groups are
-// numbered tile by tile (8x8 pixels) and light by light, so the 64 groups a
-// wave takes together are one light's rays from one 8x8 tile. A lane whose
-// group is finished takes the next group from a global counter (one atomic
-// per wave for all lanes that need work), so lanes stay busy until the frame
-// runs out of groups instead of idling beside pixels with more rays.
+// Shadow pass, persistent waves with dynamic fetch, over one frame or a batch
+// of frames (each with its own scene, G-buffer and lights). The work unit is a
+// group = (frame, pixel, light): its rays are traced by one lane in sample
+// order and its shadow factor is written once by that lane. Groups are
+// numbered frame by frame, then tile by tile (tile_w x 64/tile_w pixels), then
+// light by light, so the 64 groups a wave takes together are one light's rays
+// from one tile. A lane whose group is finished takes the next group from a
+// global counter (one atomic per wave for all lanes that need work), so lanes
+// stay busy until the launch runs out of groups. With BATCH = false the single
+// frame's job is read with uniform (scalar) loads.
 // Traversal: near-first stack walk, internal nodes in an inner loop until the
 // lane reaches a leaf, then the leaf's triangles.
-template <bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBufView G, float* shadow,
-                                                   uint32_t* next_group, uint32_t ngroups,
+template <bool COUNT, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                   uint32_t ngroups, uint32_t* next_group,
                                                    unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     uint32_t* lds = stk + threadIdx.x;
@@ -101,15 +103,15 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
     Cnt c = {};
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    const size_t plane = (size_t)F.width * F.height;
-    const uint32_t SS = F.samples;
-    const uint32_t tiles_x = (F.width + 7u) / 8u;
-    const uint32_t per_tile = F.nl * 64u;
     // current group
-    size_t p = 0;
-    uint32_t li = 0, seed = 0, s = 0, unocc = 0;
-    uint32_t target = SS;   // rays of the current phase
-    bool refine = false;    // adaptive: the 64-sample phase after disagreeing probes
+    uint32_t job = 0;
+    const TNode* nodes = jobs[0].S.nodes;
+    const STri* tris = jobs[0].S.tris;
+    uint32_t node_count = jobs[0].S.node_count;
+    float* out = nullptr;
+    uint32_t li = 0, seed = 0, s = 0, unocc = 0, SS = 1;
+    uint32_t target = 1;    // rays of the current phase
+    bool refine = false;    // adaptive: the penumbra phase after disagreeing probes
     v3 P = mk(0.0f, 0.0f, 0.0f), L = P;
     float dist = 0.0f;
     bool isDir = false, in_group = false;
@@ -128,13 +130,14 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
                     if (hit) ++c.occluded; else ++unocc;
                 }
                 if (s == target) {
-                    if (F.adaptive && !refine && target == 4u && unocc != 0u && unocc != 4u) {
+                    bool adaptive = jobs[job].F.adaptive != 0u;
+                    if (adaptive && !refine && target == 4u && unocc != 0u && unocc != 4u && SS > 4u) {
                         refine = true;  // the probes disagree: a penumbra
-                        target = 64u;
+                        target = SS;
                         s = 0;
                         unocc = 0;
                     } else {
-                        shadow[li * plane + p] = (float)unocc / (float)target;
+                        *out = (float)unocc / (float)target;
                         in_group = false;
                     }
                 }
@@ -154,17 +157,31 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
                         exhausted = true;
                         break;
                     }
-                    uint32_t t = g / per_tile, rem = g - t * per_tile;
+                    if (BATCH) {  // the frame holding group g
+                        uint32_t lo = 0, hi = njobs;
+                        while (hi - lo > 1) {
+                            uint32_t mid = (lo + hi) >> 1;
+                            if (jobs[mid].group_base <= g) lo = mid; else hi = mid;
+                        }
+                        job = lo;
+                    }
+                    const ShadowJob& J = jobs[job];
+                    const FrameArgs& F = J.F;
+                    uint32_t lg = g - J.group_base;
+                    uint32_t per_tile = F.nl * 64u;
+                    uint32_t t = lg / per_tile, rem = lg - t * per_tile;
                     li = rem >> 6;
                     uint32_t q = rem & 63u;
-                    uint32_t x = (t % tiles_x) * 8u + (q & 7u), y = (t / tiles_x) * 8u + (q >> 3);
+                    uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+                    uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
                     if (x >= F.width || y >= F.height) continue;
-                    p = (size_t)y * F.width + x;
-                    float4 P4 = G.pos[p], N4 = G.nrm[p];
+                    size_t p = (size_t)y * F.width + x;
+                    out = J.shadow + li * ((size_t)F.width * F.height) + p;
+                    float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
                     bool background = P4.w == 0.0f && N4.w == 0.0f;
-                    int mode = (int)S.mats[mat_index(S, P4.w)].shading_metal_rough_idx[0];
+                    int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
                     if (background || mode == 4) {
-                        shadow[li * plane + p] = background ? 0.0f : 1.0f;
+                        *out = background ? 0.0f : 1.0f;
                         continue;
                     }
                     P = ld3(P4);
@@ -176,23 +193,30 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
                     L = normalize(L);
                     float NdotL = clamp01(dot(N, L));
                     if (!(NdotL > 0.0f)) {
-                        shadow[li * plane + p] = 0.0f;
+                        *out = 0.0f;
                         continue;
                     }
                     if (!(isDir || dist < lt.pos_maxdist[3])) {
-                        shadow[li * plane + p] = 1.0f;
+                        *out = 1.0f;
                         continue;
                     }
                     seed = tea(y * F.width + x, F.frame);
+                    SS = F.samples;
                     s = 0;
                     unocc = 0;
                     refine = false;
                     target = F.adaptive ? (hard_light(F, li) ? 1u : 4u) : SS;
+                    if (BATCH) {
+                        nodes = J.S.nodes;
+                        tris = J.S.tris;
+                        node_count = J.S.node_count;
+                    }
                     in_group = true;
                     break;
                 }
                 if (exhausted) break;
             }
+            const FrameArgs& F = jobs[job].F;
             v3 dir;
             if (F.adaptive) {
                 Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (refine ? 64u + s : s));
@@ -207,11 +231,11 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
             ++s;
             hit = false;
             sp = 0;
-            cur = S.node_count ? 0u : DONE;
+            cur = node_count ? 0u : DONE;
             continue;
         }
         while (cur < LEAF_BIT) {
-            const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
+            const float4* np = reinterpret_cast<const float4*>(nodes + cur);
             float4 a = np[0], b = np[1], cc = np[2], d = np[3];
             if (COUNT) ++c.nodes;
             float tn0, tn1;
@@ -244,7 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(DevScene S, FrameArgs F, GBuf
             uint32_t first = cur & 0x07FFFFFFu;
             uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
             for (uint32_t k = first; k < first + cnt; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                const float4* tp = reinterpret_cast<const float4*>(tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 if (COUNT) ++c.tris;
                 float t, b1, b2;
@@ -317,15 +341,21 @@ hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, un
     else k_gbuffer<false><<<pixel_blocks(f), BLOCK, 0, st>>>(s, f, gb, counters);
     return hipGetLastError();
 }
-hipError_t launch_shadows(const DevScene& s, const FrameArgs& f, GBufView gb, float* shadow,
-                          unsigned long long* counters, bool count, hipStream_t st, uint32_t* next_group,
-                          uint32_t resident_blocks) {
-    uint32_t tiles = ((f.width + 7u) / 8u) * ((f.height + 7u) / 8u);
-    uint32_t ngroups = tiles * f.nl * 64u;
+uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w) {
+    uint32_t tw = tile_w, th = 64u / tile_w;
+    uint32_t tiles = ((f.width + tw - 1u) / tw) * ((f.height + th - 1u) / th);
+    return tiles * f.nl * 64u;
+}
+hipError_t launch_shadows(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, unsigned long long* counters,
+                          bool count, hipStream_t st, uint32_t* next_group, uint32_t resident_blocks) {
     uint32_t nb = min(resident_blocks, (ngroups + BLOCK - 1) / BLOCK);
+    if (nb == 0) return hipSuccess;
     k_reset_u32<<<1, 64, 0, st>>>(next_group);
-    if (count) k_shadows<true><<<nb, BLOCK, 0, st>>>(s, f, gb, shadow, next_group, ngroups, counters);
-    else k_shadows<false><<<nb, BLOCK, 0, st>>>(s, f, gb, shadow, next_group, ngroups, counters);
+    bool batch = njobs > 1;
+    if (count && batch) k_shadows<true, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, ngroups, next_group, counters);
+    else if (count) k_shadows<true, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, ngroups, next_group, counters);
+    else if (batch) k_shadows<false, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, ngroups, next_group, counters);
+    else k_shadows<false, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, ngroups, next_group, counters);
     return hipGetLastError();
 }
 hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits,

@@ -75,6 +75,13 @@ def load(path=LIB_PATH):
         "hr_shade_hybrid": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
                                  C.POINTER(abi.hr_trace_params), vp, vp, vp, C.POINTER(abi.hr_stats)]),
         "hr_trace_rays": (st, [vp, vp, u32, i32, vp, C.POINTER(abi.hr_stats)]),
+        "hr_trace_shadows_batch": (st, [vp, vp, u32, C.POINTER(abi.hr_stats)]),
+        "hr_shade_hybrid_batch": (st, [vp, vp, u32, C.POINTER(abi.hr_stats)]),
+        "hr_graph_capture": (st, [vp, vp, u32, C.POINTER(vp)]),
+        "hr_graph_launch": (st, [vp, C.POINTER(abi.hr_stats)]),
+        "hr_graph_destroy": (None, [vp]),
+        "hr_context_set_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
+        "hr_context_get_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -113,6 +120,15 @@ class Context:
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def set_tuning(self, blocks_per_cu=8, tile_w=8, wave_budget_mb=3072):
+        t = abi.hr_tuning(blocks_per_cu, tile_w, wave_budget_mb, 0)
+        self.check(self.L.hr_context_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = abi.hr_tuning()
+        self.check(self.L.hr_context_get_tuning(self.h, C.byref(t)))
+        return dict(blocks_per_cu=t.blocks_per_cu, tile_w=t.tile_w, wave_budget_mb=t.wave_budget_mb)
 
 
 class DeviceBuffer:
@@ -306,3 +322,82 @@ class HybridRenderer:
 
     def download_refl(self):
         return self.refl.download((self.height, self.width, 4))
+
+
+def _merge_stats(st):
+    d = st.as_dict()
+    return {k: (int(v) if isinstance(v, int) else float(v)) for k, v in d.items()}
+
+
+class BatchRenderer:
+    """Several frames (HybridRenderers with their own scenes, G-buffers and
+    outputs) traced together: one persistent shadow kernel over all frames,
+    then the shading pass of each; or the whole batch captured in a hipGraph."""
+
+    def __init__(self, ctx, renderers):
+        self.ctx = ctx
+        self.renderers = list(renderers)
+        self._build_descs()
+
+    def _build_descs(self):
+        n = len(self.renderers)
+        self.descs = (abi.hr_frame_desc * n)()
+        for i, rd in enumerate(self.renderers):
+            d = self.descs[i]
+            d.scene = rd.hs.h.value
+            d.gbuffer = rd.gb.c
+            d.lights = C.cast(rd.lights, C.c_void_p)
+            d.shapes = None if rd.shapes is None else C.cast(rd.shapes, C.c_void_p)
+            d.light_count = rd.nl
+            d.camera = rd.cam
+            d.params = rd.params
+            d.shadow = rd.shadow.ptr
+            d.lit = rd.lit.ptr
+            d.refl = rd.refl.ptr
+
+    def gbuffers(self):
+        for rd in self.renderers:
+            rd.gbuffer_primary()
+
+    def trace_shadows(self, stats=False):
+        st = abi.hr_stats() if stats else None
+        self.ctx.check(self.ctx.L.hr_trace_shadows_batch(self.ctx.h, C.cast(self.descs, C.c_void_p),
+                                                         len(self.renderers), C.byref(st) if st else None))
+        return _merge_stats(st) if st else None
+
+    def shade(self, stats=False):
+        st = abi.hr_stats() if stats else None
+        self.ctx.check(self.ctx.L.hr_shade_hybrid_batch(self.ctx.h, C.cast(self.descs, C.c_void_p),
+                                                        len(self.renderers), C.byref(st) if st else None))
+        return _merge_stats(st) if st else None
+
+    def capture(self):
+        return Graph(self)
+
+
+class Graph:
+    """A batch frame (shadows + shading) captured once and replayed."""
+
+    def __init__(self, batch):
+        self.batch = batch
+        self.ctx = batch.ctx
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.hr_graph_capture(self.ctx.h, C.cast(batch.descs, C.c_void_p),
+                                                   len(batch.renderers), C.byref(h)))
+        self.h = h
+
+    def launch(self, stats=False):
+        st = abi.hr_stats() if stats else None
+        self.ctx.check(self.ctx.L.hr_graph_launch(self.h, C.byref(st) if st else None))
+        return _merge_stats(st) if st else None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.hr_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

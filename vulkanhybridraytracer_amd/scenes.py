@@ -389,6 +389,29 @@ def make_1080p_4m(seed=0, **kw):
     return make_terrain_scene(seed, quads=1024, instances=1024, extent=400.0, name="hybrid_1080p_4m", **kw)
 
 
+BATCH_RESOLUTIONS = ((256, 256), (640, 360), (1280, 720), (1920, 1080))
+
+
+def make_batch(count=32, seed_base=0, tri_range=(50_000, 2_000_000), resolutions=BATCH_RESOLUTIONS, spp=16):
+    """hybrid_batch32: `count` scenes of the terrain generator with seeds
+    seed_base.., triangle counts log-uniform in tri_range (half terrain, half
+    2048-triangle soups), resolutions drawn from `resolutions`, adaptive
+    sampling up to `spp` (shadow penumbra samples and glossy paths),
+    roughness U[0,1]."""
+    out = []
+    for k in range(count):
+        rng = np.random.default_rng(10_000 + seed_base + k)
+        tris = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        quads = max(8, int(round(np.sqrt(tris / 4.0))))
+        instances = max(1, int(round(tris / 2.0 / 2048.0)))
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        extent = 200.0 * quads / 512.0 + 40.0
+        out.append(make_terrain_scene(seed=seed_base + k, width=w, height=h, quads=quads, instances=instances,
+                                      extent=extent, samples=spp, spp=spp, rough_max=1.0, adaptive=True,
+                                      name="batch%02d" % k))
+    return out
+
+
 WORKLOADS = {
     "hybrid_640x360_20k": make_small,
     "hybrid_1080p_1m": make_1080p_1m,
