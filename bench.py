@@ -98,6 +98,83 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def denoise_measure(steps=3, log_fn=log):
+    """4 spp + denoise against plain 64 spp on the 1920x1080, 4.2M-triangle
+    scene (non-adaptive sampling): ms per frame of each, their ratio, the
+    rMSE and max abs error of the denoised frame (and of the raw 4 spp frame)
+    against the 64 spp frame, and the denoiser's MFMA roofline."""
+    from vulkanhybridraytracer_amd import abi, denoise, hr, scenes
+    if not os.path.exists(denoise.WEIGHTS_PATH):
+        return {"skipped": "no trained weights at %s" % os.path.relpath(denoise.WEIGHTS_PATH, ROOT)}
+    sc = scenes.make_1080p_4m(samples=4, spp=4, adaptive=False)
+    ctx = hr.Context(0)
+    hs = hr.HybridScene(ctx, sc)
+    rd = hr.HybridRenderer(ctx, hs)
+    rd.gbuffer_primary()
+    W, H = sc.width, sc.height
+    low = dict(sc.params, shadow_samples=4, refl_spp=4, flags=0)
+    high = dict(sc.params, shadow_samples=64, refl_spp=64, flags=0)
+
+    def frame(params, reps):
+        rd.set_frame(sc.camera, sc.lights, sc.shapes, params)
+        ms, rays = [], 0
+        for _ in range(reps):
+            s1, s2 = abi.hr_stats(), abi.hr_stats()
+            rd.trace_shadows(s1)
+            rd.shade_hybrid(s2)
+            ms.append(s1.ms_shadow + s2.ms_shade)
+            rays = s1.shadow_rays + s2.reflection_rays + s2.hit_shadow_rays
+        return float(np.median(ms)), int(rays)
+
+    ms4, rays4 = frame(low, steps + 1)
+    noisy = rd.lit.download((H, W, 4))
+    dn = denoise.Denoiser.load(ctx)
+    out = ctx.alloc(W * H * 16)
+    noisy_buf = ctx.alloc(W * H * 16)
+    noisy_buf.upload(noisy)
+    img = denoise.Denoiser.image(rd.gb, noisy_buf.ptr, None, out.ptr, _eye(sc))
+    dn.denoise(img)
+    ms_dn = float(np.median([dn.denoise(img, timing=True) for _ in range(steps + 2)]))
+    den = out.download((H, W, 4))
+    ms64, rays64 = frame(high, 1)
+    ref = rd.lit.download((H, W, 4))
+    flops = denoise.flops(H, W)
+    tflops = flops / (ms_dn * 1e-3) / 1e12
+    res = {"workload": "denoise_1080p_4m", "ms_4spp": round(ms4, 3), "ms_denoise": round(ms_dn, 3),
+           "ms_4spp_plus_denoise": round(ms4 + ms_dn, 3), "ms_64spp": round(ms64, 3),
+           "speedup_vs_64spp": round(ms64 / (ms4 + ms_dn), 2),
+           "rmse_denoised": round(denoise.rmse(den, ref), 5), "rmse_4spp": round(denoise.rmse(noisy, ref), 5),
+           "max_abs_denoised": round(float(np.abs(den[..., :3] - ref[..., :3]).max()), 4),
+           "max_abs_4spp": round(float(np.abs(noisy[..., :3] - ref[..., :3]).max()), 4),
+           "rays_4spp": rays4, "rays_64spp": rays64,
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tflops / BF16_PEAK_TFLOPS, 4), "flops_per_frame": int(flops),
+                        "kernel": "denoiser forward (k_conv + features + kernel prediction)"}}
+    log_fn("denoise", res)
+    dn.close()
+    ctx.close()
+    return res
+
+
+def _eye(sc):
+    """Camera position: translation column of the inverse view matrix."""
+    iv = np.asarray(sc.camera[:16], dtype=np.float64).reshape(4, 4)  # column-major
+    return tuple(float(v) for v in iv[3, :3])
+
+
+def denoise_main(args):
+    res = denoise_measure(args.steps)
+    out = {"metric": "ms/frame 4spp+denoise (1080p, 4M tris)", "value": res.get("ms_4spp_plus_denoise"),
+           "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res.get("ms_4spp_plus_denoise"), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16", "data": "synthetic", "config": {"workload": "denoise_1080p_4m"},
+           **{k: v for k, v in res.items() if k != "workload"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +187,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes before timing (batch workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true",
+                    help="skip the 4 spp + denoise vs 64 spp measurement of the default line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -123,6 +202,9 @@ def main():
 
     if args.workload == "hybrid_batch32":
         batch_main(args)
+        return
+    if args.workload == "denoise_1080p_4m":
+        denoise_main(args)
         return
     from vulkanhybridraytracer_amd import hr, scenes
 
@@ -210,6 +292,10 @@ def main():
            "rays_per_step": per_step, "ms_shadow": round(avg_shadow_ms, 3),
            "ms_shade": round(float(np.mean(ms_shade)), 3), "ms_gbuffer": round(gst["ms_gbuffer"], 3),
            "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu}
+    if sc.name == "hybrid_1080p_1m" and not args.no_denoise:
+        del rd, hs
+        ctx.close()
+        out["denoise"] = denoise_measure(3)
     print(json.dumps(out), flush=True)
 
 

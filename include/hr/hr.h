@@ -352,6 +352,75 @@ typedef struct hr_tuning {
 hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
 hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
 
+/* ---- learned denoiser (no counterpart in the reference, whose only
+ * "denoise" is the temporal shadow filter blur.comp) --------------------------
+ * A kernel-predicting U-Net: 5 levels of two 3x3 convolutions with ReLU
+ * (32, 48, 64, 64, 64 channels), 2x2 average pooling, nearest upsampling with
+ * the encoder output concatenated in front, and a 3x3 head predicting 25
+ * logits per pixel; the softmax of the logits weights the 5x5 neighbourhood
+ * (clamped at the frame border) of the noisy lit colour. Input features per
+ * pixel: albedo (3), normal (3), distance to the eye / 100 (1), roughness (1),
+ * log(1 + luminance of the noisy colour) (1). BF16 activations and weights,
+ * FP32 accumulation; FP32 master weights and Adam state.
+ *
+ * Weights are exchanged as FP32 in layer order e0a e0b e1a e1b e2a e2b e3a
+ * e3b e4a e4b d3a d3b d2a d2b d1a d1b d0a d0b head, each as W[cout][3][3][cin]
+ * (tap (dy + 1) * 3 + dx + 1) followed by b[cout]; hr_denoiser_layer gives
+ * the shapes. */
+typedef struct hr_denoiser hr_denoiser;
+#define HR_DN_LAYERS 19
+uint64_t hr_denoiser_param_count(void);
+/* Shape of layer i: input / output channels and pyramid level (0 = full resolution). */
+hr_status hr_denoiser_layer(uint32_t i, uint32_t* cin, uint32_t* cout, uint32_t* level);
+/* weights may be NULL: He-normal initialisation from seed. */
+hr_status hr_denoiser_create(hr_context* ctx, const float* weights, uint64_t count, uint64_t seed, hr_denoiser** out);
+void hr_denoiser_destroy(hr_denoiser* dn);
+hr_status hr_denoiser_get_weights(hr_denoiser* dn, float* weights, uint64_t count);
+/* Replaces the weights and resets the optimiser state. */
+hr_status hr_denoiser_set_weights(hr_denoiser* dn, const float* weights, uint64_t count);
+
+/* One image: its G-buffer (position, normal, albedo and material planes are
+ * read), the noisy lit colour, and for training the reference colour. All
+ * pointers are device float4[height][width]. */
+typedef struct hr_dn_image {
+    hr_gbuffer gbuffer;
+    const float* noisy;
+    const float* target; /* training only, else NULL */
+    float* out;          /* hr_denoise only */
+    float eye[3];
+    float _reserved;
+} hr_dn_image;
+
+/* Training loss, per pixel and colour channel, averaged:
+ *   HR_DN_LOSS_L1_TONEMAPPED  |T(out) - T(target)|, T(v) = max(v, 0) / (1 + max(v, 0))
+ *   HR_DN_LOSS_REL_MSE        (out - target)^2 / (target^2 + 0.01)  (the rMSE metric) */
+#define HR_DN_LOSS_L1_TONEMAPPED 0u
+#define HR_DN_LOSS_REL_MSE 1u
+hr_status hr_denoiser_set_loss(hr_denoiser* dn, uint32_t loss);
+/* Denoises one frame into image->out; ms (optional) receives the device time. */
+hr_status hr_denoise(hr_denoiser* dn, const hr_dn_image* image, float* ms);
+/* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on n images of one
+ * size for the loss chosen by hr_denoiser_set_loss (default tone-mapped L1).
+ * loss (optional) receives the loss before the step. */
+hr_status hr_denoiser_train_step(hr_denoiser* dn, const hr_dn_image* images, uint32_t n, float lr, float* loss);
+/* Loss and FP32 weight gradients (weight exchange layout, on the host) of n
+ * images without updating the weights. */
+hr_status hr_denoiser_gradients(hr_denoiser* dn, const hr_dn_image* images, uint32_t n, float* loss, float* grad,
+                                uint64_t count);
+
+/* One 3x3 convolution layer (the denoiser's building block) on device BF16
+ * NHWC tensors, zero padding. weight: host FP32 W[cout][3][3][cin], bias:
+ * host FP32 [cout] or NULL. cin must be a multiple of 16.
+ *   forward:  y[n][h][w][cout] = relu?(conv(x, W) + b)
+ *   backward: dx = gradient of conv(x, W) w.r.t. x for output gradient dy,
+ *             dweight / dbias (host FP32, W's layout) the weight and bias
+ *             gradients; cout must be a multiple of 16. */
+hr_status hr_dn_conv3x3_forward(hr_context* ctx, const void* x, uint32_t n, uint32_t h, uint32_t w, uint32_t cin,
+                                const float* weight, const float* bias, uint32_t cout, int relu, void* y);
+hr_status hr_dn_conv3x3_backward(hr_context* ctx, const void* x, const void* dy, uint32_t n, uint32_t h, uint32_t w,
+                                 uint32_t cin, const float* weight, uint32_t cout, void* dx, float* dweight,
+                                 float* dbias);
+
 #ifdef __cplusplus
 }
 #endif

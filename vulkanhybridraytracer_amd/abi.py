@@ -64,6 +64,11 @@ class hr_gbuffer(C.Structure):
                 ("emissive", C.c_void_p)]
 
 
+class hr_dn_image(C.Structure):
+    _fields_ = [("gbuffer", hr_gbuffer), ("noisy", C.c_void_p), ("target", C.c_void_p), ("out", C.c_void_p),
+                ("eye", C.c_float * 3), ("_reserved", C.c_float)]
+
+
 class hr_trace_params(C.Structure):
     _fields_ = [("shadow_samples", C.c_uint32), ("refl_spp", C.c_uint32),
                 ("max_depth", C.c_uint32), ("flags", C.c_uint32)]

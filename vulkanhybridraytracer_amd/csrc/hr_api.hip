@@ -10,25 +10,9 @@
 #include <cstring>
 #include <vector>
 
-#include "internal.h"
+#include "api_internal.h"
 
 using namespace hr;
-
-struct hr_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    unsigned long long* d_counters = nullptr;
-    uint32_t* d_next_group = nullptr;  // work counter of the persistent shadow kernel
-    uint32_t cu_count = 256;
-    uint32_t resident_blocks = 2048;    // persistent grid: cu_count x tuning.blocks_per_cu
-    hr_tuning tuning{8u, 8u, 3072u, 0u};
-    void* work = nullptr;  // grow-only workspace of the reflection pipeline
-    size_t work_bytes = 0;
-    ShadowJob* d_jobs = nullptr;  // grow-only device copy of the shadow jobs of a call
-    uint32_t jobs_cap = 0;
-    std::string err;
-};
 
 struct hr_scene {
     hr_context* ctx = nullptr;
@@ -53,27 +37,6 @@ struct hr_scene {
 
 namespace {
 
-hr_status fail(hr_context* ctx, hr_status code, const char* fmt, ...) {
-    if (ctx) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        ctx->err = buf;
-    }
-    return code;
-}
-hr_status hip_fail(hr_context* ctx, hipError_t e, const char* what) {
-    hr_status code = (e == hipErrorOutOfMemory) ? HR_E_OOM : HR_E_DEVICE;
-    return fail(ctx, code, "%s: %s", what, hipGetErrorString(e));
-}
-#define HR_HIP(ctx, call)                                   \
-    do {                                                    \
-        hipError_t e_ = (call);                             \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
-    } while (0)
-
 template <class T>
 hr_status scene_alloc(hr_scene* s, T** p, size_t count) {
     size_t bytes = count * sizeof(T);
@@ -85,12 +48,6 @@ hr_status scene_alloc(hr_scene* s, T** p, size_t count) {
     *p = static_cast<T*>(q);
     return HR_OK;
 }
-#define HR_TRY(x)                     \
-    do {                              \
-        hr_status st_ = (x);          \
-        if (st_ != HR_OK) return st_; \
-    } while (0)
-
 // transpose(inverse(mat3(model))) as cofactors / determinant, row-major
 // (hybridHit.rchit:50); evaluated as written (no contraction)
 void normal_matrix(const float* m, float* nm) {

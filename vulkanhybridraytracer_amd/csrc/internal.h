@@ -173,4 +173,77 @@ size_t radix_hist_words(uint32_t n);
 hipError_t lbvh_build(BuildBuffers& b, hipStream_t st);   // full build
 hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st);   // new models, same topology
 
+// ---- denoiser kernels (denoise.hip) ------------------------------------------------
+// Activations are BF16 NHWC images of N x H x W pixels; a tensor is a channel
+// slice [off, off + count) of a buffer with C channels per pixel.
+// Weights of a 3x3 layer, GEMM form W[co][k], k = tap * Cin + ci, tap =
+// (dy + 1) * 3 + (dx + 1), are stored in MFMA fragment order: element (co, k)
+// at ((k / 32 * (Cout_pad / 16) + co / 16) * 64 + (co % 16) + (k % 32) / 8 * 16) * 8 + k % 8.
+struct ConvArgs {
+    const __bf16* in;
+    const __bf16* wt;      // fragment order, Cout_pad x Kpad
+    const float* bias;     // Cout or null
+    __bf16* out;           // BF16 output (when out_f32 is null)
+    float* out_f32;        // FP32 output instead
+    uint32_t N, H, W;
+    uint32_t in_C, in_off, Cin, K, Kpad;  // K = 9 * Cin, Kpad = K rounded up to 32
+    uint32_t out_C, out_off, Cout, Cout_pad;
+    uint32_t relu;
+};
+struct WgradArgs {
+    const __bf16* dout;    // gradient of the layer output (after the ReLU mask)
+    const __bf16* in;      // layer input
+    float* partial;        // chunks x Cout_pad x Kpad
+    float* bpartial;       // chunks x Cout_pad
+    uint32_t N, H, W;
+    uint32_t dout_C, dout_off, in_C, in_off;
+    uint32_t Cin, K, Kpad, Cout_pad, chunk;
+};
+struct FeatArgs {
+    const float4 *pos, *nrm, *alb, *mat, *noisy;
+    __bf16* x0;            // N x Hp x Wp x 16
+    float cam[3];
+    uint32_t W, H, Wp, Hp;
+};
+struct KpcnArgs {
+    const float* logits;   // N x Hp x Wp x 32 (25 used)
+    __bf16* dlogits;
+    const float4* noisy;   // image n, W x H
+    const float4* target;
+    float4* out;
+    uint32_t W, H, Wp, Hp;
+    uint32_t loss;         // 0 tone-mapped L1, 1 relative MSE
+};
+struct AdamArgs {
+    float *w, *grad, *m, *v;
+    size_t n;
+    float lr, b1, b2, eps, bc1, bc2;
+};
+struct PackArgs {
+    const float* w;        // FP32 master, Cout_pad x Kpad, row-major
+    __bf16* wf;            // forward weights, fragment order
+    __bf16* wd;            // input-gradient weights (flipped taps), fragment order, Cin x Kpad_d
+    uint32_t Cin, Cout_pad, Kpad, Kpad_d;
+};
+hipError_t dn_conv(const ConvArgs& a, hipStream_t st);
+uint32_t dn_wgrad_chunks(uint32_t P, uint32_t* chunk);
+hipError_t dn_wgrad(const WgradArgs& a, float* wgrad, float* bgrad, hipStream_t st);
+hipError_t dn_relu_bwd(__bf16* g, uint32_t g_C, uint32_t g_off, const __bf16* act, uint32_t a_C, uint32_t a_off,
+                       uint32_t C, uint32_t P, hipStream_t st);
+hipError_t dn_pool(const __bf16* in, uint32_t in_C, uint32_t in_off, __bf16* out, uint32_t C, uint32_t N, uint32_t Ho,
+                   uint32_t Wo, hipStream_t st);
+hipError_t dn_pool_bwd_add(const __bf16* gout, __bf16* gin, uint32_t gin_C, uint32_t gin_off, uint32_t C, uint32_t N,
+                           uint32_t Ho, uint32_t Wo, hipStream_t st);
+hipError_t dn_up(const __bf16* in, __bf16* out, uint32_t out_C, uint32_t out_off, uint32_t C, uint32_t N, uint32_t Ho,
+                 uint32_t Wo, hipStream_t st);
+hipError_t dn_up_bwd(const __bf16* gout, uint32_t g_C, uint32_t g_off, __bf16* gin, uint32_t C, uint32_t N, uint32_t Hi,
+                     uint32_t Wi, hipStream_t st);
+hipError_t dn_features(const FeatArgs& a, uint32_t n, hipStream_t st);
+hipError_t dn_kpcn(const KpcnArgs& a, uint32_t n, hipStream_t st);
+uint32_t dn_kpcn_blocks(uint32_t W, uint32_t H);
+hipError_t dn_kpcn_bwd(const KpcnArgs& a, uint32_t n, float scale, float* loss_partial, hipStream_t st);
+hipError_t dn_fill_bf16(__bf16* p, size_t n, hipStream_t st);
+hipError_t dn_adam(const AdamArgs& a, hipStream_t st);
+hipError_t dn_pack(const PackArgs& a, hipStream_t st);
+
 }  // namespace hr

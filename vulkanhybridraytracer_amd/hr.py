@@ -12,9 +12,11 @@ The classes mirror the reference's host objects for the hybrid pass:
 The library must be built (python -m vulkanhybridraytracer_amd.build); there
 is no fallback: every call goes to the HIP kernels.
 """
+import atexit
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -34,10 +36,23 @@ class HRError(RuntimeError):
 def exported_functions():
     """Names of the functions declared in include/hr/hr.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:hr_status|void|uint32_t|const char\*)\s+(hr_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:hr_status|void|uint32_t|uint64_t|const char\*)\s+(hr_\w+)\s*\(", text, re.M)))
 
 
 _lib = None
+_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_contexts():
+    """Release every live context (children first) before the interpreter
+    tears down modules, so no library object is destroyed in an arbitrary
+    order during finalisation."""
+    for c in list(_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
 
 
 def load(path=LIB_PATH):
@@ -48,7 +63,7 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError("libhr.so is not built: run python -m vulkanhybridraytracer_amd.build")
     L = C.CDLL(path)
-    vp, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
+    vp, u32, i32, sz, u64 = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t, C.c_uint64
     st = C.c_int
     sigs = {
         "hr_abi_version": (u32, []),
@@ -82,6 +97,18 @@ def load(path=LIB_PATH):
         "hr_graph_destroy": (None, [vp]),
         "hr_context_set_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
         "hr_context_get_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
+        "hr_denoiser_param_count": (u64, []),
+        "hr_denoiser_layer": (st, [u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
+        "hr_denoiser_create": (st, [vp, vp, u64, u64, C.POINTER(vp)]),
+        "hr_denoiser_destroy": (None, [vp]),
+        "hr_denoiser_get_weights": (st, [vp, vp, u64]),
+        "hr_denoiser_set_weights": (st, [vp, vp, u64]),
+        "hr_denoiser_set_loss": (st, [vp, u32]),
+        "hr_denoise": (st, [vp, C.POINTER(abi.hr_dn_image), C.POINTER(C.c_float)]),
+        "hr_denoiser_train_step": (st, [vp, vp, u32, C.c_float, C.POINTER(C.c_float)]),
+        "hr_denoiser_gradients": (st, [vp, vp, u32, C.POINTER(C.c_float), vp, u64]),
+        "hr_dn_conv3x3_forward": (st, [vp, vp, u32, u32, u32, u32, vp, vp, u32, i32, vp]),
+        "hr_dn_conv3x3_backward": (st, [vp, vp, vp, u32, u32, u32, u32, vp, u32, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -99,6 +126,11 @@ class Context:
         if rc != 0:
             raise HRError(rc, "hr_context_create(%d) failed (no HIP device?)" % device)
         self.h = h
+        self._children = weakref.WeakSet()  # scenes, graphs, denoisers: closed before the context
+        _contexts.add(self)
+
+    def own(self, obj):
+        self._children.add(obj)
 
     def check(self, rc):
         if rc != 0:
@@ -106,6 +138,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for c in list(self._children):
+                c.close()
             self.L.hr_context_destroy(self.h)
             self.h = None
 
@@ -183,6 +217,7 @@ class HybridScene:
                                        C.cast(self.packed.materials, C.c_void_p), self.packed.material_count,
                                        C.byref(bp), C.byref(h)))
         self.h = h
+        ctx.own(self)
         ctx.check(ctx.L.hr_scene_set_env(h, C.byref(self.packed.env)))
 
     def close(self):
@@ -385,6 +420,7 @@ class Graph:
         self.ctx.check(self.ctx.L.hr_graph_capture(self.ctx.h, C.cast(batch.descs, C.c_void_p),
                                                    len(batch.renderers), C.byref(h)))
         self.h = h
+        self.ctx.own(self)
 
     def launch(self, stats=False):
         st = abi.hr_stats() if stats else None
