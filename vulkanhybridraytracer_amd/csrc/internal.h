@@ -16,6 +16,7 @@ constexpr uint32_t STACK_LDS = 16;       // entries per lane kept in LDS
 constexpr uint32_t STACK_SPILL = MAX_DEPTH - STACK_LDS;  // entries per lane in private memory
 constexpr uint32_t LEAF_MAX_LIMIT = 16;  // leaf code holds count - 1 in 4 bits
 constexpr uint32_t MAX_TRIS = 1u << 27;  // leaf code holds the first triangle in 27 bits
+constexpr uint32_t SHADOW_QUEUES = 8;    // shadow work bands, one per XCD (counters in a 64-byte buffer)
 
 __host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) { return LEAF_BIT | ((count - 1u) << 27) | first; }
 
@@ -111,8 +112,9 @@ struct ShadowJob {
     uint32_t _pad;
 };
 uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w);
-// jobs_dev: njobs ShadowJob in device memory; next_group: one device uint32
-// (reset by the launch); resident_blocks: the persistent grid size
+// jobs_dev: njobs ShadowJob in device memory; next_group: SHADOW_QUEUES device
+// uint32 band counters (reset by the launch); resident_blocks: the persistent
+// grid size
 hipError_t launch_shadows(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, unsigned long long* counters,
                           bool count, hipStream_t st, uint32_t* next_group, uint32_t resident_blocks);
 // Workspace of the wavefront reflection pipeline for one slab of pixels

@@ -5,11 +5,12 @@
 //   k_shadows   shadowRaygen.rgen:28-93 (+ shadow.rmiss, shadowHit.rchit)
 //   k_trace     bare traceRayEXT queries
 //
-// One lane per pixel; a 256-thread workgroup covers 16x16 pixels and each
-// 64-lane wave an 8x8 tile, so a wave's rays start close together and walk
-// similar nodes. Workgroups are renumbered so that consecutive screen tiles
-// land on the same XCD (dispatch assigns workgroup i to XCD i mod 8), which
-// keeps a screen region's working set in one L2.
+// k_gbuffer and k_trace: one lane per pixel (ray); a 256-thread workgroup
+// covers 16x16 pixels and each 64-lane wave an 8x8 tile, so a wave's rays start
+// close together and walk similar nodes. Workgroups are renumbered so that
+// consecutive screen tiles land on the same XCD (dispatch assigns workgroup i
+// to XCD i mod 8), which keeps a screen region's working set in one L2.
+// k_shadows is persistent and fetches its work dynamically (see below).
 //
 // Traversal: 64-byte nodes holding both child boxes (four 16-byte loads),
 // near child first, far child pushed on a per-lane stack whose first 16
@@ -87,9 +88,12 @@ __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBuf
 // order and its shadow factor is written once by that lane. Groups are
 // numbered frame by frame, then tile by tile (tile_w x 64/tile_w pixels), then
 // light by light, so the 64 groups a wave takes together are one light's rays
-// from one tile. A lane whose group is finished takes the next group from a
-// global counter (one atomic per wave for all lanes that need work), so lanes
-// stay busy until the launch runs out of groups. With BATCH = false the single
+// from one tile. The group list is cut into SHADOW_QUEUES contiguous bands
+// (screen bands), one per XCD, each with its own counter. A lane whose group
+// is finished takes the next group from its wave's current band (one atomic
+// per wave for all lanes that need work): first the band of its XCD, so
+// neighbouring tiles share that XCD's L2, then the other bands in turn, so
+// lanes stay busy until every band is empty. With BATCH = false the single
 // frame's job is read with uniform (scalar) loads.
 // Traversal: near-first stack walk, internal nodes in an inner loop until the
 // lane reaches a leaf, then the leaf's triangles.
@@ -103,6 +107,10 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
     Cnt c = {};
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // Workgroups are dispatched round-robin over the XCDs, so workgroup b runs
+    // on XCD b mod 8; its home band is the one of that XCD.
+    const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
+    uint32_t emptied = 0;  // bands this lane has seen empty, in visiting order
     // current group
     uint32_t job = 0;
     const TNode* nodes = jobs[0].S.nodes;
@@ -143,19 +151,34 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
                 }
             }
             if (!in_group) {
-                // take groups until one has rays to trace
+                // take groups until one has rays to trace: from the home queue
+                // first, then from the others in turn
                 bool exhausted = false;
                 for (;;) {
+                    // Bands below a lane's count are known to be empty, so the
+                    // wave uses the largest count among its active lanes: one
+                    // band for every lane of this fetch, the one the leader's
+                    // atomic counts in.
+                    uint32_t k = 0;
+                    for (uint32_t b = 1; b <= SHADOW_QUEUES; ++b)
+                        if (__ballot(emptied >= b)) k = b;
+                    emptied = k;
+                    if (k >= SHADOW_QUEUES) {
+                        exhausted = true;
+                        break;
+                    }
+                    const uint32_t band = (home + k) & (SHADOW_QUEUES - 1u);
+                    const uint32_t band_end = (uint32_t)(((uint64_t)ngroups * (band + 1u)) / SHADOW_QUEUES);
                     unsigned long long need = __ballot(1);
                     uint32_t rank = (uint32_t)__popcll(need & lt_mask);
                     uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
                     uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(next_group, (uint32_t)__popcll(need));
+                    if (lane == leader) base = atomicAdd(next_group + band, (uint32_t)__popcll(need));
                     base = (uint32_t)__shfl((int)base, (int)leader, 64);
-                    uint32_t g = base + rank;
-                    if (g >= ngroups) {
-                        exhausted = true;
-                        break;
+                    uint32_t g = (uint32_t)(((uint64_t)ngroups * band) / SHADOW_QUEUES) + base + rank;
+                    if (g >= band_end) {  // the band is empty: move to the next one
+                        emptied = k + 1u;
+                        continue;
                     }
                     if (BATCH) {  // the frame holding group g
                         uint32_t lo = 0, hi = njobs;
@@ -319,7 +342,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DevScene S, const hr_ray* rays,
 }
 
 __global__ void k_reset_u32(uint32_t* p) {
-    if (threadIdx.x == 0) *p = 0u;
+    if (threadIdx.x < SHADOW_QUEUES) p[threadIdx.x] = 0u;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
