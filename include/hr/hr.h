@@ -352,6 +352,40 @@ typedef struct hr_tuning {
 hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
 hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
 
+/* ---- temporal shadow filter ----------------------------------------------
+ * hr_filter_shadows replaces the compute pass blur.comp (dispatch
+ * renderer.cpp:2661-2675, descriptor set renderer.cpp:2030-2082) that runs
+ * between the shadow and hybrid passes; the hybrid pass then reads its output
+ * (renderer.cpp:3086-3090): pass `history` as the shadow buffer of
+ * hr_shade_hybrid. Any width, height and light count (the reference
+ * hard-codes 1700x900 and 3 lights, blur.comp:7-10) and every pixel (the
+ * reference dispatches W/16 x H/16 whole workgroups).
+ *   shadow   device float[light_count][H][W], this frame's traced factors
+ *   history  device float[light_count][H][W], the previous output, replaced by
+ *            this frame's (the reference's _denoisedImages)
+ *   frame    the FrameCount value (vk_engine.cpp:669-686, 722): -1 is set on a
+ *            view change and incremented before use, so the first frame after
+ *            a change has frame 0
+ *   motion   device float2[H][W] = (ndc - ndc_prev) * 0.5 + 0.5 as the raster
+ *            G-buffer stores it (geometry_shader.frag:72-75), read at the
+ *            pixel's own texel; required when frame <= 0, else may be NULL
+ * frame > 0: history = mix(history, shadow, 1 / (frame + 1)).
+ * frame <= 0: the previous output at the reprojected pixel (x + r.x * W,
+ * y + r.x * H), r = motion * 2 - 1 (the y offset uses r.x as blur.comp:52
+ * evaluates it), truncated toward zero, 0 outside the frame, clamped to the
+ * 3x3 min/max of this frame's factors (0 outside the frame), then
+ * mix(old, shadow, 0.4). Reprojected reads see the previous output (the
+ * reference reads and writes one image in place). */
+hr_status hr_filter_shadows(hr_context* ctx, uint32_t width, uint32_t height, uint32_t light_count, int32_t frame,
+                            const float* shadow, const float* motion, float* history);
+/* Motion vectors of a G-buffer for the filter: for each pixel with geometry,
+ * (ndc - ndc_prev).xy * 0.5 + 0.5 of its world position under view_proj and
+ * prev_view_proj (projection * view, column-major 4x4, as basic.vert:38-46
+ * uses them); 0 for background pixels (skybox.frag:19). motion: device
+ * float2[H][W]. */
+hr_status hr_motion_vectors(hr_context* ctx, const hr_gbuffer* gbuffer, const float* view_proj,
+                            const float* prev_view_proj, float* motion);
+
 /* ---- learned denoiser (no counterpart in the reference, whose only
  * "denoise" is the temporal shadow filter blur.comp) --------------------------
  * A kernel-predicting U-Net: 5 levels of two 3x3 convolutions with ReLU

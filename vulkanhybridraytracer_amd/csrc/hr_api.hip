@@ -963,4 +963,41 @@ hr_status hr_graph_launch(hr_graph* g, hr_stats* stats) {
     return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 
+hr_status hr_filter_shadows(hr_context* ctx, uint32_t width, uint32_t height, uint32_t nl, int32_t frame,
+                            const float* shadow, const float* motion, float* history) {
+    if (!ctx) return HR_E_INVALID;
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(ctx, HR_E_INVALID, "frame size %ux%u out of range", width, height);
+    if (nl == 0 || nl > HR_MAX_LIGHTS) return fail(ctx, HR_E_INVALID, "light_count %u out of range", nl);
+    if (!shadow || !history) return fail(ctx, HR_E_INVALID, "shadow and history must be device pointers");
+    if (frame <= 0 && !motion) return fail(ctx, HR_E_INVALID, "motion vectors are needed when frame <= 0");
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    const float* prev = history;
+    if (frame <= 0) {  // reprojected reads must see the previous output
+        const size_t bytes = (size_t)nl * width * height * sizeof(float);
+        HR_TRY(ensure_work(ctx, bytes));
+        HR_HIP(ctx, hipMemcpyAsync(ctx->work, history, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        prev = static_cast<const float*>(ctx->work);
+    }
+    HR_HIP(ctx, launch_filter_shadows(width, height, nl, frame, shadow, motion, prev, history, ctx->stream));
+    return HR_OK;
+}
+
+hr_status hr_motion_vectors(hr_context* ctx, const hr_gbuffer* gb, const float* view_proj,
+                            const float* prev_view_proj, float* motion) {
+    if (!ctx) return HR_E_INVALID;
+    HR_TRY(check_gbuffer(ctx, gb));
+    if (!view_proj || !prev_view_proj || !motion)
+        return fail(ctx, HR_E_INVALID, "view_proj, prev_view_proj and motion are required");
+    MotionArgs a;
+    for (int i = 0; i < 16; ++i) {
+        a.vp[i] = view_proj[i];
+        a.vp_prev[i] = prev_view_proj[i];
+    }
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, launch_motion(gb->width, gb->height, reinterpret_cast<const float4*>(gb->position),
+                              reinterpret_cast<const float4*>(gb->normal), a, motion, ctx->stream));
+    return HR_OK;
+}
+
 }  // extern "C"

@@ -140,6 +140,17 @@ hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, 
                              unsigned long long* counters, bool count, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
 
+// ---- temporal shadow filter and motion vectors (temporal.hip) ----------------
+struct MotionArgs {
+    float vp[16];       // projection * view of this frame, column-major
+    float vp_prev[16];  // of the previous frame
+};
+// shadow, prev, out: float[nl][H][W]; motion: float2[H][W] (read when frame <= 0)
+hipError_t launch_filter_shadows(uint32_t W, uint32_t H, uint32_t nl, int32_t frame, const float* shadow,
+                                 const float* motion, const float* prev, float* out, hipStream_t st);
+hipError_t launch_motion(uint32_t W, uint32_t H, const float4* pos, const float4* nrm, const MotionArgs& a,
+                         float* motion, hipStream_t st);
+
 // ---- LBVH build (lbvh.hip) ------------------------------------------------------
 struct BuildBuffers {
     // inputs

@@ -97,6 +97,8 @@ def load(path=LIB_PATH):
         "hr_graph_destroy": (None, [vp]),
         "hr_context_set_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
         "hr_context_get_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
+        "hr_filter_shadows": (st, [vp, u32, u32, u32, C.c_int32, vp, vp, vp]),
+        "hr_motion_vectors": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, vp]),
         "hr_denoiser_param_count": (u64, []),
         "hr_denoiser_layer": (st, [u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]),
         "hr_denoiser_create": (st, [vp, vp, u64, u64, C.POINTER(vp)]),
@@ -357,6 +359,31 @@ class HybridRenderer:
 
     def download_refl(self):
         return self.refl.download((self.height, self.width, 4))
+
+    # temporal shadow filter (blur.comp) between the shadow and hybrid passes
+    def filter_shadows(self, history, frame, motion=None):
+        """history: DeviceBuffer of float[nl][H][W] (previous output, replaced);
+        motion: DeviceBuffer of float2[H][W], needed when frame <= 0."""
+        self.ctx.check(self.ctx.L.hr_filter_shadows(self.ctx.h, self.width, self.height, self.nl, int(frame),
+                                                    self.shadow.ptr, None if motion is None else motion.ptr,
+                                                    history.ptr))
+
+    def motion_vectors(self, view_proj, prev_view_proj, motion):
+        """motion vectors of the current G-buffer into a float2[H][W] buffer;
+        view_proj / prev_view_proj: 16 floats, column-major projection * view"""
+        vp = np.ascontiguousarray(view_proj, dtype=np.float32)
+        pv = np.ascontiguousarray(prev_view_proj, dtype=np.float32)
+        self.ctx.check(self.ctx.L.hr_motion_vectors(self.ctx.h, C.byref(self.gb.c), vp.ctypes.data, pv.ctypes.data,
+                                                    motion.ptr))
+
+    def shade_hybrid_with(self, shadow, stats=None):
+        """hr_shade_hybrid reading the given shadow buffer (e.g. the filter's
+        history, as renderer.cpp:3086-3090 binds the filtered images)"""
+        L = self.ctx.L
+        self.ctx.check(L.hr_shade_hybrid(self.hs.h, C.byref(self.gb.c), C.cast(self.lights, C.c_void_p),
+                                         self._shapes_ptr(), self.nl, C.byref(self.cam), C.byref(self.params),
+                                         shadow.ptr, self.lit.ptr, self.refl.ptr,
+                                         C.byref(stats) if stats is not None else None))
 
 
 def _merge_stats(st):
