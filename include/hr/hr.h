@@ -352,6 +352,36 @@ typedef struct hr_tuning {
 hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
 hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
 
+/* ---- raster G-buffer ingest ------------------------------------------------
+ * The offscreen attachments of the reference's raster G-buffer pass
+ * (renderer.cpp:252-261, written by geometry_shader.frag:66-75 and
+ * skybox.frag:16-21), device pointers, row-major, tightly packed:
+ *   position  R16G16B16A16_SFLOAT  (world xyz, material index)
+ *   normal    R16G16B16A16_SFLOAT  (0.5 n + 0.5, 1; 0 on background)
+ *   albedo    R8G8B8A8_UNORM       (diffuse rgb before linearisation)
+ *   motion    R16G16_SFLOAT        ((ndc - ndc_prev) * 0.5 + 0.5), may be NULL
+ *   material  R8G8B8A8_UNORM       (0, roughness, metallic)
+ *   emissive  R8G8B8A8_UNORM
+ * hr_gbuffer_ingest decodes them into the FP32 planes exactly as the shaders
+ * read them (shadowRaygen.rgen:36-41, hybridRaygen.rgen:40-47): half to
+ * float, normal = n * 2 - 1 (w kept), UNORM8 = c / 255; motion_out (device
+ * float2[H][W], may be NULL) receives the motion vectors for
+ * hr_filter_shadows. hr_gbuffer_quantize is the inverse the raster pass
+ * performs: half by round to nearest even, UNORM8 = rint(clamp(v, 0, 1) *
+ * 255), normal * 0.5 + 0.5 (0 on background). */
+typedef struct hr_raster_gbuffer {
+    uint32_t width, height;
+    void* position;
+    void* normal;
+    void* albedo;
+    void* motion;
+    void* material;
+    void* emissive;
+} hr_raster_gbuffer;
+hr_status hr_gbuffer_ingest(hr_context* ctx, const hr_raster_gbuffer* in, const hr_gbuffer* out, float* motion_out);
+hr_status hr_gbuffer_quantize(hr_context* ctx, const hr_gbuffer* in, const float* motion_in,
+                              const hr_raster_gbuffer* out);
+
 /* ---- temporal shadow filter ----------------------------------------------
  * hr_filter_shadows replaces the compute pass blur.comp (dispatch
  * renderer.cpp:2661-2675, descriptor set renderer.cpp:2030-2082) that runs

@@ -64,6 +64,11 @@ class hr_gbuffer(C.Structure):
                 ("emissive", C.c_void_p)]
 
 
+class hr_raster_gbuffer(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("position", C.c_void_p), ("normal", C.c_void_p),
+                ("albedo", C.c_void_p), ("motion", C.c_void_p), ("material", C.c_void_p), ("emissive", C.c_void_p)]
+
+
 class hr_dn_image(C.Structure):
     _fields_ = [("gbuffer", hr_gbuffer), ("noisy", C.c_void_p), ("target", C.c_void_p), ("out", C.c_void_p),
                 ("eye", C.c_float * 3), ("_reserved", C.c_float)]

@@ -983,6 +983,35 @@ hr_status hr_filter_shadows(hr_context* ctx, uint32_t width, uint32_t height, ui
     return HR_OK;
 }
 
+hr_status hr_gbuffer_ingest(hr_context* ctx, const hr_raster_gbuffer* in, const hr_gbuffer* out, float* motion_out) {
+    if (!ctx) return HR_E_INVALID;
+    if (!in || !in->position || !in->normal || !in->albedo || !in->material || !in->emissive)
+        return fail(ctx, HR_E_INVALID, "raster G-buffer planes must be non-NULL device pointers");
+    HR_TRY(check_gbuffer(ctx, out));
+    if (in->width != out->width || in->height != out->height)
+        return fail(ctx, HR_E_INVALID, "raster G-buffer %ux%u does not match %ux%u", in->width, in->height,
+                    out->width, out->height);
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, launch_gbuffer_ingest(in->width * in->height, in->position, in->normal, in->albedo, in->motion,
+                                      in->material, in->emissive, gview(out), motion_out, ctx->stream));
+    return HR_OK;
+}
+
+hr_status hr_gbuffer_quantize(hr_context* ctx, const hr_gbuffer* in, const float* motion_in,
+                              const hr_raster_gbuffer* out) {
+    if (!ctx) return HR_E_INVALID;
+    HR_TRY(check_gbuffer(ctx, in));
+    if (!out || !out->position || !out->normal || !out->albedo || !out->material || !out->emissive)
+        return fail(ctx, HR_E_INVALID, "raster G-buffer planes must be non-NULL device pointers");
+    if (in->width != out->width || in->height != out->height)
+        return fail(ctx, HR_E_INVALID, "raster G-buffer %ux%u does not match %ux%u", out->width, out->height,
+                    in->width, in->height);
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, launch_gbuffer_quantize(in->width * in->height, gview(in), motion_in, out->position, out->normal,
+                                        out->albedo, out->motion, out->material, out->emissive, ctx->stream));
+    return HR_OK;
+}
+
 hr_status hr_motion_vectors(hr_context* ctx, const hr_gbuffer* gb, const float* view_proj,
                             const float* prev_view_proj, float* motion) {
     if (!ctx) return HR_E_INVALID;

@@ -151,6 +151,12 @@ hipError_t launch_filter_shadows(uint32_t W, uint32_t H, uint32_t nl, int32_t fr
 hipError_t launch_motion(uint32_t W, uint32_t H, const float4* pos, const float4* nrm, const MotionArgs& a,
                          float* motion, hipStream_t st);
 
+// ---- raster G-buffer formats (gbuffer_io.hip) --------------------------------
+hipError_t launch_gbuffer_ingest(uint32_t n, const void* pos, const void* nrm, const void* alb, const void* mot,
+                                 const void* mat, const void* emi, GBufView G, float* motion_out, hipStream_t st);
+hipError_t launch_gbuffer_quantize(uint32_t n, GBufView G, const float* motion_in, void* pos, void* nrm, void* alb,
+                                   void* mot, void* mat, void* emi, hipStream_t st);
+
 // ---- LBVH build (lbvh.hip) ------------------------------------------------------
 struct BuildBuffers {
     // inputs

@@ -97,6 +97,8 @@ def load(path=LIB_PATH):
         "hr_graph_destroy": (None, [vp]),
         "hr_context_set_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
         "hr_context_get_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
+        "hr_gbuffer_ingest": (st, [vp, C.POINTER(abi.hr_raster_gbuffer), C.POINTER(abi.hr_gbuffer), vp]),
+        "hr_gbuffer_quantize": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, C.POINTER(abi.hr_raster_gbuffer)]),
         "hr_filter_shadows": (st, [vp, u32, u32, u32, C.c_int32, vp, vp, vp]),
         "hr_motion_vectors": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, vp]),
         "hr_denoiser_param_count": (u64, []),
@@ -285,6 +287,44 @@ class GBuffer:
     def upload(self, planes):
         for k in PLANES:
             self.bufs[k].upload(np.ascontiguousarray(planes[k], dtype=np.float32))
+
+    def ingest(self, raster, motion_out=None):
+        """decode a RasterGBuffer (the reference's attachment formats) into
+        these FP32 planes; motion_out: DeviceBuffer of float2[H][W] or None"""
+        self.ctx.check(self.ctx.L.hr_gbuffer_ingest(self.ctx.h, C.byref(raster.c), C.byref(self.c),
+                                                    None if motion_out is None else motion_out.ptr))
+
+    def quantize(self, raster, motion_in=None):
+        """encode these planes into a RasterGBuffer as the raster pass writes them"""
+        self.ctx.check(self.ctx.L.hr_gbuffer_quantize(self.ctx.h, C.byref(self.c),
+                                                      None if motion_in is None else motion_in.ptr,
+                                                      C.byref(raster.c)))
+
+
+# attachment formats of renderer.cpp:252-261: (numpy dtype, components)
+RASTER_PLANES = {"position": (np.float16, 4), "normal": (np.float16, 4), "albedo": (np.uint8, 4),
+                 "motion": (np.float16, 2), "material": (np.uint8, 4), "emissive": (np.uint8, 4)}
+
+
+class RasterGBuffer:
+    """The reference's raster G-buffer attachments (RGBA16F position and
+    normal, RGBA8 albedo / material / emissive, RG16F motion) in device memory."""
+
+    def __init__(self, ctx, width, height, with_motion=True):
+        self.ctx, self.width, self.height = ctx, width, height
+        self.bufs = {k: ctx.alloc(width * height * np.dtype(t).itemsize * n) for k, (t, n) in RASTER_PLANES.items()
+                     if with_motion or k != "motion"}
+        self.c = abi.hr_raster_gbuffer(width, height, *[self.bufs[k].ptr if k in self.bufs else None
+                                                        for k in RASTER_PLANES])
+
+    def download(self):
+        return {k: self.bufs[k].download((self.height, self.width, RASTER_PLANES[k][1]), RASTER_PLANES[k][0])
+                for k in self.bufs}
+
+    def upload(self, planes):
+        for k in self.bufs:
+            t, n = RASTER_PLANES[k]
+            self.bufs[k].upload(np.ascontiguousarray(planes[k], dtype=t))
 
 
 class HybridRenderer:
