@@ -352,6 +352,24 @@ typedef struct hr_tuning {
 hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
 hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
 
+/* ---- pure ray tracing mode ---------------------------------------------------
+ * The reference's RAYTRACING render mode: raygen.rgen + closesthit.rchit +
+ * miss.rmiss + shadow.rmiss, dispatched by build_raytracing_command_buffers
+ * (renderer.cpp:2572-2616) and submitted by Renderer::raytrace
+ * (renderer.cpp:465-528). Per pixel, params->refl_spp camera samples (the
+ * first through the pixel centre, raygen.rgen:27-33; NSAMPLES in the
+ * reference), each a closest-hit chain of at most params->max_depth rays
+ * (MAX_RECURSION) shaded by closesthit.rchit with params->shadow_samples
+ * shadow rays per light at every hit (closesthit.rchit:99-121; a light
+ * beyond its range, and a directional light, stays shadowed there).
+ * accum: device float4[height][width]; camera->frame[0] < 0 stores the new
+ * colour, otherwise accum = mix(accum, colour, 1 / (1 + frame))
+ * (raygen.rgen:69-80). stats: primary_rays, reflection_rays (the chain's
+ * later rays), hit_shadow_rays, occluded_rays, ms_shade. */
+hr_status hr_render_rt(hr_scene* scene, const hr_light* lights, const hr_light_shape* shapes, uint32_t light_count,
+                       const hr_camera* camera, const hr_trace_params* params, uint32_t width, uint32_t height,
+                       float* accum, hr_stats* stats);
+
 /* ---- raster G-buffer ingest ------------------------------------------------
  * The offscreen attachments of the reference's raster G-buffer pass
  * (renderer.cpp:252-261, written by geometry_shader.frag:66-75 and

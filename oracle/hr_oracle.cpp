@@ -221,7 +221,10 @@ enum : uint32_t {
     PURPOSE_REFLECT = 2,        /* index = spp sample (GGX lobe at the G-buffer surface) */
     PURPOSE_HIT_SHADOW = 3,     /* index = (sample * max_depth + depth) * nl + light */
     PURPOSE_HIT_REFLECT = 4,    /* index = sample * max_depth + depth */
-    PURPOSE_ADAPTIVE_SHADOW = 5 /* index = light * 128 + k (k < 4 probes, 64 + s refinement) */
+    PURPOSE_ADAPTIVE_SHADOW = 5, /* index = light * 128 + k (k < 4 probes, 64 + s refinement) */
+    PURPOSE_RT_PIXEL = 6,        /* pure ray tracing: index = pixel sample (sub-pixel offset) */
+    PURPOSE_RT_SHADOW = 7,       /* pure ray tracing: index = (sample * max_depth + depth) * nl + light */
+    PURPOSE_RT_REFLECT = 8       /* pure ray tracing: index = sample * max_depth + depth */
 };
 static inline Pcg ray_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index) {
     return pcg_init((uint64_t)pixel_seed, ((uint64_t)purpose << 32) | (uint64_t)index);
@@ -1058,6 +1061,130 @@ static HitOut shade_hit(const or_scene& s, const Frame& f, v3 ro, v3 rd, const H
     return out;
 }
 
+/* closesthit.rchit:24-186 for one closest hit of the pure ray tracing mode:
+ * `samples` shadow rays per light (closesthit.rchit:99-121; a light beyond
+ * its range, and a directional light, stays shadowed), the factor is the
+ * unshadowed fraction; attenuation without a directional case; the glossy
+ * extension (GGX lobe) for shading model 3 with roughness > 0. */
+static HitOut shade_rt(const or_scene& s, const Frame& f, v3 ro, v3 rd, const Hit& hit, uint32_t seed,
+                       uint32_t sample, uint32_t depth, Cnt& c) {
+    HitOut out;
+    float b1 = hit.u, b2 = hit.v;
+    float b0 = 1.0f - b1 - b2;
+    const Prim& p = s.prims[s.tri_prim[hit.tri]];
+    uint32_t local = hit.tri - p.tri_offset;
+    uint32_t ib = s.ibase[p.mesh] + p.first_index + 3 * local;
+    const hr_vertex& x0 = s.verts[s.vbase[p.mesh] + s.idx[ib + 0]];
+    const hr_vertex& x1 = s.verts[s.vbase[p.mesh] + s.idx[ib + 1]];
+    const hr_vertex& x2 = s.verts[s.vbase[p.mesh] + s.idx[ib + 2]];
+    v3 nrm = add(add(scl(ld3(x0.normal), b0), scl(ld3(x1.normal), b1)), scl(ld3(x2.normal), b2));
+    const float* M = p.nm;
+    v3 N = normalize(mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z, (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
+                        (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z));
+    v3 V = normalize(neg(rd));
+    float NdotV = clamp01(dot(N, V));
+    v3 worldPos = add(ro, scl(rd, hit.t));
+    v3 Lo = mk(0, 0, 0);
+    const hr_material& mat = s.mats[p.material];
+    int mode = (int)mat.shading_metal_rough_idx[0];
+    v3 albedo = pow3(ld3(mat.diffuse_ior), 2.2f);
+    float roughness = mat.shading_metal_rough_idx[2];
+    float metallic = mat.shading_metal_rough_idx[1];
+    v3 F0 = mixs(mk(0.04f, 0.04f, 0.04f), albedo, metallic);
+    v3 irradiance = env_lookup(s.irr, N, PI_H);
+    v3 direction = mk(1, 1, 1);
+    bool scatter = false;
+    v3 reflected = mk(0, 0, 0);
+    if (mode == 3) {
+        v3 Nr = N;
+        if (roughness > 0.0f) {
+            Pcg g = ray_stream(seed, PURPOSE_RT_REFLECT, sample * f.max_depth + depth);
+            float r1 = u01(g), r2 = u01(g);
+            Nr = sampleGGX(N, roughness, r1, r2);
+        }
+        reflected = reflect(normalize(rd), Nr);
+    }
+    uint32_t SS = f.samples;
+    for (uint32_t i = 0; i < f.nl; ++i) {
+        const hr_light& lt = f.lights[i];
+        bool isDir = lt.pos_maxdist[3] < 0.0f;
+        v3 L = isDir ? ld3(lt.pos_maxdist) : sub(ld3(lt.pos_maxdist), worldPos);
+        float maxd = lt.pos_maxdist[3];
+        float dist = length(L);
+        L = normalize(L);
+        float intensity = isDir ? 1.0f : (lt.color_intensity[3] / (dist * dist));
+        v3 H = normalize(add(V, L));
+        float NdotL = clamp01(dot(N, L));
+        float NdotH = clamp01(dot(N, H));
+        float shadowFactor = 0.0f;
+        if (NdotL > 0.0f) {
+            Pcg gs = ray_stream(seed, PURPOSE_RT_SHADOW, (sample * f.max_depth + depth) * f.nl + i);
+            for (uint32_t a = 0; a < SS; ++a) {
+                bool shadowed = true;
+                if (dist < maxd) {
+                    v3 dir = light_sample_dir(f, i, worldPos, L, gs, a, SS);
+                    ++c.hitshadow;
+                    shadowed = occluded(s, add(worldPos, scl(dir, 1e-2f)), dir, 0.001f, dist + 1.0f, c);
+                    if (shadowed) ++c.occluded;
+                }
+                if (!shadowed) shadowFactor += 1.0f;
+            }
+            shadowFactor /= (float)SS;
+        }
+        float attenuation;
+        if (intensity == 0.0f) {
+            attenuation = 0.0f;
+        } else {
+            attenuation = maxd - dist;
+            attenuation /= maxd;
+            attenuation = fmaxf(attenuation, 0.0f);
+            attenuation = attenuation * attenuation;
+        }
+        v3 lc = ld3(lt.color_intensity);
+        if (mode == 0) {
+            v3 radiance = scl(scl(scl(lc, intensity), attenuation), shadowFactor);
+            v3 F = FresnelSchlick(NdotH, F0);
+            float D = DistributionGGX(N, H, roughness);
+            float G = GeometrySmith(N, V, L, roughness);
+            v3 numerator = scl(F, D * G);
+            float denominator = fmaxf(4.0f * clamp01(dot(N, V)) * NdotL, 0.000001f);
+            v3 specular = divs(numerator, denominator);
+            v3 kD = scl(sub(mk(1, 1, 1), F), 1.0f - metallic);
+            v3 term = add(divs(mul(kD, albedo), PI_H), specular);
+            Lo = add(Lo, scl(mul(term, radiance), NdotL));
+            direction = mk(1, 1, 1);
+            scatter = false;
+        } else if (mode == 3) {
+            bool sc = dot(reflected, N) > 0.0f;
+            v3 t = (NdotL > 0.0f && intensity > 0.0f)
+                       ? mul(scl(scl(scl(scl(lc, intensity), attenuation), shadowFactor), metallic), albedo)
+                       : scl(mul(irradiance, albedo), metallic);
+            Lo = add(Lo, t);
+            direction = reflected;
+            scatter = sc;
+        } else if (mode == 4) {
+            float ior = mat.diffuse_ior[3];
+            float ndv = dot(N, normalize(rd));
+            v3 refrNormal = ndv > 0.0f ? neg(N) : N;
+            float refrEta = ndv > 0.0f ? 1.0f / ior : ior;
+            v3 t = (intensity > 0.0f) ? scl(mul(scl(scl(lc, intensity), attenuation), albedo), metallic)
+                                      : scl(mul(irradiance, albedo), metallic);
+            Lo = add(Lo, t);
+            float radicand = 1.0f + refrEta * refrEta * (ndv * ndv - 1.0f);
+            direction = radicand < 0.0f ? reflect(rd, N) : refract(rd, refrNormal, refrEta);
+            scatter = true;
+        }
+    }
+    v3 F = FresnelSchlick(NdotV, F0);
+    v3 kD = scl(sub(mk(1, 1, 1), F), 1.0f - metallic);
+    v3 ambient = mul(mul(kD, albedo), irradiance);
+    out.color = add(Lo, ambient);
+    out.dir = direction;
+    out.scatter = scatter;
+    out.origin = worldPos;
+    return out;
+}
+
 } // namespace
 
 /* ======================================================================== */
@@ -1514,4 +1641,66 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
     to_counters(total, cnt);
 }
 
+
+/* raygen.rgen:20-81: camera samples, the closest-hit chain, accumulation */
+void or_render_rt(const or_scene* s, const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                  const hr_camera* cam, const hr_trace_params* prm, uint32_t W, uint32_t H, float* accum,
+                  uint32_t row_start, uint32_t row_step, or_counters* cnt, uint32_t threads) {
+    Cnt total;
+    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam), false};
+    const int frame_signed = (int)cam->frame[0];
+    parallel_rows(H, row_start, row_step, threads, [&](uint32_t y, Cnt& c) {
+        for (uint32_t x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            uint32_t seed = tea(y * W + x, f.frame);
+            v3 finalColor = mk(0, 0, 0);
+            uint32_t spp = std::max(f.spp, 1u);
+            for (uint32_t nS = 0; nS < spp; ++nS) {
+                float ox = 0.5f, oy = 0.5f;
+                if (nS > 0) {
+                    Pcg gp = ray_stream(seed, PURPOSE_RT_PIXEL, nS);
+                    ox = u01(gp);
+                    oy = u01(gp);
+                }
+                float ux = ((float)x + ox) / (float)W;
+                float uy = ((float)y + oy) / (float)H;
+                float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+                float org[4], tgt[4], dir[4];
+                mul4(cam->inv_view, 0.0f, 0.0f, 0.0f, 1.0f, org);
+                mul4(cam->inv_proj, dx, dy, 1.0f, 1.0f, tgt);
+                v3 tn = normalize(mk(tgt[0], tgt[1], tgt[2]));
+                mul4(cam->inv_view, tn.x, tn.y, tn.z, 0.0f, dir);
+                v3 o = mk(org[0], org[1], org[2]), d = mk(dir[0], dir[1], dir[2]);
+                v3 rayColor = mk(1, 1, 1);
+                for (uint32_t depth = 0; depth < f.max_depth; ++depth) {
+                    v3 ro = add(o, scl(d, 1e-2f));
+                    RayP r = prep_ray(ro, d, 0.001f, 10000.0f);
+                    Hit h;
+                    if (depth == 0) ++c.primary; else ++c.refl;
+                    if (!traverse(*s, r, false, h, c)) {
+                        rayColor = mul(rayColor, pow3(env_lookup(s->sky, normalize(d), PI_M), 2.2f));
+                        break;
+                    }
+                    HitOut hc = shade_rt(*s, f, ro, d, h, seed, nS, depth, c);
+                    rayColor = mul(rayColor, hc.color);
+                    if (!hc.scatter) break;
+                    o = hc.origin;
+                    d = hc.dir;
+                }
+                finalColor = add(finalColor, rayColor);
+            }
+            finalColor = divs(finalColor, (float)spp);
+            float* out = accum + 4 * p;
+            if (frame_signed < 0) {
+                out[0] = finalColor.x; out[1] = finalColor.y; out[2] = finalColor.z;
+            } else {
+                float a = 1.0f / (1.0f + (float)frame_signed);
+                v3 m = add(scl(mk(out[0], out[1], out[2]), 1.0f - a), scl(finalColor, a));
+                out[0] = m.x; out[1] = m.y; out[2] = m.z;
+            }
+            out[3] = 1.0f;
+        }
+    }, total);
+    to_counters(total, cnt);
+}
 } /* extern "C" */

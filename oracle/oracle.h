@@ -88,6 +88,10 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H,
                      const hr_camera* cam, const hr_trace_params* params, const float* shadow,
                      float* lit, float* refl, uint32_t row_start, uint32_t row_step,
                      or_counters* cnt, uint32_t threads);
+/* pure ray tracing mode (raygen.rgen, closesthit.rchit): accum float4[H][W] */
+void or_render_rt(const or_scene* s, const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                  const hr_camera* cam, const hr_trace_params* params, uint32_t W, uint32_t H, float* accum,
+                  uint32_t row_start, uint32_t row_step, or_counters* cnt, uint32_t threads);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,8 @@ def lib():
         L.or_shade_hybrid.argtypes = [vp, u32, u32, vp, vp, vp, vp, vp, vp, vp, u32,
                                       C.POINTER(abi.hr_camera), C.POINTER(abi.hr_trace_params), vp, vp,
                                       vp, u32, u32, C.POINTER(or_counters), u32]
+        L.or_render_rt.argtypes = [vp, vp, vp, u32, C.POINTER(abi.hr_camera), C.POINTER(abi.hr_trace_params), u32,
+                                   u32, vp, u32, u32, C.POINTER(or_counters), u32]
         _lib = L
     return _lib
 
@@ -208,6 +210,25 @@ class OracleScene:
                               len(lt), C.byref(cam), C.byref(prm), shadow.ctypes.data, lit.ctypes.data,
                               refl.ctypes.data, rows[0], rows[1], C.byref(cnt), self.threads)
         return lit, refl, cnt.as_dict()
+
+    def render_rt(self, accum=None, camera=None, params=None, rows=(0, 1), lights=None, shapes="scene"):
+        """pure ray tracing mode (raygen.rgen + closesthit.rchit) into accum
+        (float32 H x W x 4, updated in place; a new array when None)"""
+        sc = self.scene
+        W, H = sc.width, sc.height
+        lt = sc.lights if lights is None else lights
+        sh = sc.shapes if isinstance(shapes, str) else shapes
+        la, sa = abi.pack_lights(lt, sh)
+        cam = abi.pack_camera(sc.camera if camera is None else camera)
+        prm = self._params(params)
+        if accum is None:
+            accum = np.zeros((H, W, 4), dtype=np.float32)
+        assert accum.dtype == np.float32 and accum.flags.c_contiguous and accum.shape == (H, W, 4)
+        cnt = or_counters()
+        lib().or_render_rt(self.h, C.cast(la, C.c_void_p), None if sa is None else C.cast(sa, C.c_void_p), len(lt),
+                           C.byref(cam), C.byref(prm), W, H, accum.ctypes.data, rows[0], rows[1], C.byref(cnt),
+                           self.threads)
+        return accum, cnt.as_dict()
 
 
 # ---- small scalar hooks ----------------------------------------------------

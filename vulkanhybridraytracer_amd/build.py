@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhr.so")
-SOURCES = ["csrc/lbvh.hip", "csrc/trace.hip", "csrc/wavefront.hip", "csrc/denoise.hip", "csrc/temporal.hip", "csrc/gbuffer_io.hip", "csrc/hr_api.hip", "csrc/dn_api.hip"]
+SOURCES = ["csrc/lbvh.hip", "csrc/trace.hip", "csrc/wavefront.hip", "csrc/denoise.hip", "csrc/temporal.hip", "csrc/gbuffer_io.hip", "csrc/rt.hip", "csrc/hr_api.hip", "csrc/dn_api.hip"]
 HEADERS = ["csrc/device_math.h", "csrc/internal.h", "csrc/trace_common.h", "csrc/api_internal.h"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",

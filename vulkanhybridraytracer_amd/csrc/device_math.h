@@ -173,7 +173,10 @@ enum : uint32_t {
     PURPOSE_REFLECT = 2,         // index = spp sample
     PURPOSE_HIT_SHADOW = 3,      // index = (sample * max_depth + depth) * nl + light
     PURPOSE_HIT_REFLECT = 4,     // index = sample * max_depth + depth
-    PURPOSE_ADAPTIVE_SHADOW = 5  // index = light * 128 + k (k < 4 probes, 64 + s refinement)
+    PURPOSE_ADAPTIVE_SHADOW = 5,  // index = light * 128 + k (k < 4 probes, 64 + s refinement)
+    PURPOSE_RT_PIXEL = 6,         // pure ray tracing: index = pixel sample (sub-pixel offset)
+    PURPOSE_RT_SHADOW = 7,        // pure ray tracing: index = ((sample * max_depth + depth) * nl + light)
+    PURPOSE_RT_REFLECT = 8        // pure ray tracing: index = sample * max_depth + depth
 };
 HR_DEV Pcg ray_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index) {
     return pcg_init((uint64_t)pixel_seed, ((uint64_t)purpose << 32) | (uint64_t)index);

@@ -736,6 +736,25 @@ hr_status hr_gbuffer_primary(hr_scene* s, const hr_camera* cam, const hr_gbuffer
     return frame_end(ctx, stats, stats ? &stats->ms_gbuffer : nullptr);
 }
 
+hr_status hr_render_rt(hr_scene* s, const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                       const hr_camera* cam, const hr_trace_params* params, uint32_t width, uint32_t height,
+                       float* accum, hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (!accum) return fail(ctx, HR_E_INVALID, "accum must be a device pointer");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(ctx, HR_E_INVALID, "frame size %ux%u out of range", width, height);
+    FrameArgs f;
+    HR_TRY(fill_frame(ctx, f, cam, width, height));
+    HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
+    if (f.max_depth == 0) return fail(ctx, HR_E_INVALID, "max_depth must be at least 1");
+    if (f.adaptive) return fail(ctx, HR_E_UNSUPPORTED, "adaptive sampling is a hybrid-pass option");
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_rt(dev_scene(s), f, reinterpret_cast<float4*>(accum), (int32_t)(int)cam->frame[0],
+                          ctx->d_counters, params->flags & HR_FLAG_COUNT_VISITS, ctx->stream));
+    return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
+}
+
 hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* lights, const hr_light_shape* shapes,
                            uint32_t nl, const hr_camera* cam, const hr_trace_params* params, float* shadow,
                            hr_stats* stats) {

@@ -139,6 +139,9 @@ hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, cons
 hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits,
                              unsigned long long* counters, bool count, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+// pure ray tracing mode (rt.hip): accum float4[H][W], progressive when frame_signed >= 0
+hipError_t launch_rt(const DevScene& s, const FrameArgs& f, float4* accum, int32_t frame_signed,
+                     unsigned long long* counters, bool count, hipStream_t st);
 
 // ---- temporal shadow filter and motion vectors (temporal.hip) ----------------
 struct MotionArgs {

@@ -398,5 +398,32 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
     }
 }
 
+// Geometry of a closest hit (hybridHit.rchit:26-53, closesthit.rchit:26-54).
+struct HitGeo {
+    v3 N, V, worldPos;
+    float NdotV;
+    const hr_material* mat;
+};
+__device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, const HitD& hit) {
+    HitGeo g;
+    float b1 = hit.u, b2 = hit.v;
+    float b0 = 1.0f - b1 - b2;
+    const PrimInfo& p = S.prims[S.tri_prim[hit.tri]];
+    uint32_t local = hit.tri - p.tri_offset;
+    uint32_t ib = p.index_base + 3 * local;
+    v3 n0 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 0]]);
+    v3 n1 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 1]]);
+    v3 n2 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 2]]);
+    v3 nrm = add(add(scl(n0, b0), scl(n1, b1)), scl(n2, b2));
+    const float* M = p.nm;
+    g.N = normalize(mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z, (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
+                       (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z));
+    g.V = normalize(neg(rd));
+    g.NdotV = clamp01(dot(g.N, g.V));
+    g.worldPos = add(ro, scl(rd, hit.t));
+    g.mat = &S.mats[p.material];
+    return g;
+}
+
 }  // namespace tc
 }  // namespace hr

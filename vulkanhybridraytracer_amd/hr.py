@@ -97,6 +97,8 @@ def load(path=LIB_PATH):
         "hr_graph_destroy": (None, [vp]),
         "hr_context_set_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
         "hr_context_get_tuning": (st, [vp, C.POINTER(abi.hr_tuning)]),
+        "hr_render_rt": (st, [vp, vp, vp, u32, C.POINTER(abi.hr_camera), C.POINTER(abi.hr_trace_params), u32, u32,
+                              vp, C.POINTER(abi.hr_stats)]),
         "hr_gbuffer_ingest": (st, [vp, C.POINTER(abi.hr_raster_gbuffer), C.POINTER(abi.hr_gbuffer), vp]),
         "hr_gbuffer_quantize": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, C.POINTER(abi.hr_raster_gbuffer)]),
         "hr_filter_shadows": (st, [vp, u32, u32, u32, C.c_int32, vp, vp, vp]),
@@ -399,6 +401,17 @@ class HybridRenderer:
 
     def download_refl(self):
         return self.refl.download((self.height, self.width, 4))
+
+    # pure ray tracing mode (Renderer::raytrace, renderer.cpp:465-528)
+    def render_rt(self, accum, stats=None):
+        """raygen.rgen + closesthit.rchit into accum (DeviceBuffer of
+        float4[H][W]) with this renderer's camera, lights and parameters:
+        refl_spp camera samples, shadow_samples per light per hit, max_depth
+        rays per chain; camera frame < 0 stores, else accumulates"""
+        L = self.ctx.L
+        self.ctx.check(L.hr_render_rt(self.hs.h, C.cast(self.lights, C.c_void_p), self._shapes_ptr(), self.nl,
+                                      C.byref(self.cam), C.byref(self.params), self.width, self.height, accum.ptr,
+                                      C.byref(stats) if stats is not None else None))
 
     # temporal shadow filter (blur.comp) between the shadow and hybrid passes
     def filter_shadows(self, history, frame, motion=None):
