@@ -175,6 +175,39 @@ def denoise_main(args):
     print(json.dumps(out), flush=True)
 
 
+def rt_main(args):
+    """rt_1080p_1m: the pure ray tracing mode (raygen.rgen + closesthit.rchit)
+    on the 1080p, 1M-triangle scene: 1 camera sample per pixel, up to 8 rays
+    per chain (MAX_RECURSION), 16 stratified shadow rays per light at every
+    hit. value = camera + bounce + hit shadow rays per second."""
+    from vulkanhybridraytracer_amd import abi, hr, scenes
+    sc = scenes.make_1080p_1m()
+    ctx = hr.Context(0)
+    rd = hr.HybridRenderer(ctx, hr.HybridScene(ctx, sc))
+    rd.set_frame(sc.camera, sc.lights, sc.shapes, dict(sc.params, shadow_samples=16, refl_spp=1, max_depth=8))
+    acc = ctx.alloc(sc.width * sc.height * 16)
+    for _ in range(args.warmup):
+        rd.render_rt(acc)
+    ctx.sync()
+    rays, ms = 0, []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        st = abi.hr_stats()
+        rd.render_rt(acc, st)
+        rays += st.primary_rays + st.reflection_rays + st.hit_shadow_rays
+        ms.append(st.ms_shade)
+    ctx.sync()
+    elapsed = time.perf_counter() - t
+    out = {"metric": "Grays/s (pure ray tracing, 1080p, 1M tris)", "value": round(rays / elapsed / 1e9, 4),
+           "unit": "Grays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "rt_1080p_1m", "shadow_samples": 16, "camera_spp": 1, "max_depth": 8},
+           "ms_frame": round(float(np.mean(ms)), 3), "rays_per_step": int(rays // max(args.steps, 1))}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +238,9 @@ def main():
         return
     if args.workload == "denoise_1080p_4m":
         denoise_main(args)
+        return
+    if args.workload == "rt_1080p_1m":
+        rt_main(args)
         return
     from vulkanhybridraytracer_amd import hr, scenes
 
