@@ -131,8 +131,14 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
     const float tmin = 0.001f;
     int sp = 0;
     uint32_t spill[STACK_SPILL];
+    // Speculative traversal (Aila & Laine 2009) in the timed kernel: the
+    // shadow result does not depend on the order leaves are tested in. The
+    // counting kernel keeps the canonical near-first order, so its node and
+    // triangle counts are the oracle's.
+    constexpr bool SPEC = !COUNT;
+    uint32_t leaf = 0u;  // postponed leaf code (0: none; leaf codes have LEAF_BIT)
     for (;;) {
-        if (cur == DONE) {
+        if (cur == DONE && leaf == 0u) {
             if (in_group) {
                 if (s > 0) {  // a ray of this group just finished
                     if (hit) ++c.occluded; else ++unocc;
@@ -286,10 +292,34 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
                 --sp;
                 cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
             }
+            if (SPEC) {
+                // postpone the first leaf found and keep walking until every
+                // lane still in this loop holds one
+                if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                    leaf = cur;
+                    if (sp == 0) {
+                        cur = DONE;
+                    } else {
+                        --sp;
+                        cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                    }
+                }
+                if (__ballot(leaf == 0u) == 0ull) break;
+            }
         }
-        if (cur != DONE) {
-            uint32_t first = cur & 0x07FFFFFFu;
-            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+        // leaf phase: the postponed leaf first, else a leaf reached in order
+        uint32_t lf = DONE;
+        bool in_order = false;
+        if (SPEC && leaf != 0u) {
+            lf = leaf;
+            leaf = 0u;
+        } else if (cur >= LEAF_BIT && cur != DONE) {
+            lf = cur;
+            in_order = true;
+        }
+        if (lf != DONE) {
+            uint32_t first = lf & 0x07FFFFFFu;
+            uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
             for (uint32_t k = first; k < first + cnt; ++k) {
                 const float4* tp = reinterpret_cast<const float4*>(tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
@@ -300,11 +330,16 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
                     break;
                 }
             }
-            if (hit || sp == 0) {
+            if (hit) {  // any hit ends the ray
                 cur = DONE;
-            } else {
-                --sp;
-                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                leaf = 0u;
+            } else if (in_order) {
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
             }
         }
     }
