@@ -300,8 +300,13 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
     uint32_t best = 0xFFFFFFFFu;
     int sp = 0;
     uint32_t spill[STACK_SPILL];
+    // speculative traversal (postponed leaves) except when counting: the
+    // closest hit (smallest t, then smallest triangle id) and any-hit results
+    // do not depend on the order leaves are tested in
+    constexpr bool SPEC = !COUNT;
+    uint32_t leaf = 0u;  // postponed leaf code (0: none)
     for (;;) {
-        if (cur == DONE) {
+        if (cur == DONE && leaf == 0u) {
             if (have) {
                 HitD h;
                 h.t = tbest;
@@ -360,10 +365,31 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
                 --sp;
                 cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
             }
+            if (SPEC) {
+                if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                    leaf = cur;
+                    if (sp == 0) {
+                        cur = DONE;
+                    } else {
+                        --sp;
+                        cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                    }
+                }
+                if (__ballot(leaf == 0u) == 0ull) break;
+            }
         }
-        if (cur != DONE) {
-            uint32_t first = cur & 0x07FFFFFFu;
-            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+        uint32_t lf = DONE;
+        bool in_order = false;
+        if (SPEC && leaf != 0u) {
+            lf = leaf;
+            leaf = 0u;
+        } else if (cur >= LEAF_BIT && cur != DONE) {
+            lf = cur;
+            in_order = true;
+        }
+        if (lf != DONE) {
+            uint32_t first = lf & 0x07FFFFFFu;
+            uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
             bool stop = false;
             for (uint32_t k = first; k < first + cnt; ++k) {
                 const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
@@ -388,11 +414,16 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
                     }
                 }
             }
-            if (stop || sp == 0) {
+            if (stop) {
                 cur = DONE;
-            } else {
-                --sp;
-                cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                leaf = 0u;
+            } else if (in_order) {
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
             }
         }
     }
