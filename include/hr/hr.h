@@ -77,10 +77,13 @@ typedef struct hr_vertex {
 } hr_vertex;
 
 /* == GPUMaterial (material.h:5-10), filled as Material::materialToShader
- * (material.cpp:19-26): diffuse = (rgb, ior); textures = texture indices as
- * floats, -1 = none; shading_metal_rough_idx = (shadingModel, metallic,
- * roughness, material index). Shading models 0 (PBR), 3 (mirror/glossy) and
- * 4 (glass) exist (hybridRaygen.rgen:105-148). */
+ * (material.cpp:19-26): diffuse = (rgb, ior); textures = (diffuse, normal,
+ * emissive, metallic-roughness) indices into hr_scene_set_textures as
+ * floats, -1 = none (whole numbers only); shading_metal_rough_idx =
+ * (shadingModel, metallic, roughness, material index). Shading models 0
+ * (PBR), 3 (mirror/glossy) and 4 (glass) exist (hybridRaygen.rgen:105-148).
+ * A metallic-roughness texture gives (r, roughness = g, metallic = b), the
+ * layout the shaders read (hybridHit.rchit:65-68). */
 typedef struct hr_material {
     float diffuse_ior[4];
     float textures[4];
@@ -161,7 +164,11 @@ typedef struct hr_env {
  *   emissive = (rgb, 0)                              [gbuffers[5]]
  * Background pixels have position.w == 0 and normal.w == 0 and the sky colour
  * in albedo (skybox.frag:16-21). The normal is stored decoded (the reference
- * stores 0.5 n + 0.5 in RGBA16F, geometry_shader.frag:67). */
+ * stores 0.5 n + 0.5 in RGBA16F, geometry_shader.frag:67). With textures,
+ * hr_gbuffer_primary writes what geometry_shader.frag:54-70 writes: albedo =
+ * diffuse texel * vertex colour, the emissive texel, the metallic-roughness
+ * texel as material (w = 0), and the normal perturbed by the normal-map texel
+ * in the screen-space cotangent frame. */
 typedef struct hr_gbuffer {
     uint32_t width, height;
     float* position;
@@ -268,6 +275,21 @@ hr_status hr_scene_refit(hr_scene* scene, const float* models);
 /* Material re-upload (renderer.cpp:761-774). */
 hr_status hr_scene_set_materials(hr_scene* scene, const hr_material* materials, uint32_t count);
 hr_status hr_scene_set_env(hr_scene* scene, const hr_env* env);
+/* Material textures: the reference's Texture::_textures list (vk_textures.cpp:
+ * 128-160), bound as sampler2D[] (renderer.cpp:936-950, 3060-3070) and indexed
+ * by hr_material.textures (diffuse, normal, emissive, metallic-roughness; -1 =
+ * none). Each texture is RGBA8 as stbi_load(..., STBI_rgb_alpha) returns it
+ * (vk_textures.cpp:13-27), row-major, row 0 at v = 0, host pointer; the
+ * library copies them. Sampled as VK_FORMAT_R8G8B8A8_UNORM (c / 255) with the
+ * NEAREST / REPEAT sampler (vk_initializers.cpp:305-316): texel
+ * (floor(u * w) mod w, floor(v * h) mod h). Replaces the previous set. Every
+ * texture index the scene's materials use must be below count when a pass
+ * that shades runs (HR_E_INVALID otherwise). */
+typedef struct hr_texture {
+    const uint8_t* rgba8;
+    uint32_t width, height;
+} hr_texture;
+hr_status hr_scene_set_textures(hr_scene* scene, const hr_texture* textures, uint32_t count);
 void hr_scene_destroy(hr_scene* scene);
 
 /* Scene layout queries and device-array export (for validation). */
@@ -433,6 +455,58 @@ hr_status hr_filter_shadows(hr_context* ctx, uint32_t width, uint32_t height, ui
  * float2[H][W]. */
 hr_status hr_motion_vectors(hr_context* ctx, const hr_gbuffer* gbuffer, const float* view_proj,
                             const float* prev_view_proj, float* motion);
+
+/* ---- display ---------------------------------------------------------------
+ * The post pass (postFragment.frag:9-20, drawn by renderer.cpp's post
+ * pipeline over the final image): rgb = pow(max(rgb, 0.001), 1 / 2.2), alpha
+ * kept. in: device float4[H][W]. out (device float4[H][W], may be NULL, may
+ * equal in) receives the float result; rgba8 (device uint8[H][W][4], may be
+ * NULL) receives it stored to an 8-bit UNORM image: rint(clamp(v, 0, 1) * 255). */
+hr_status hr_post_gamma(hr_context* ctx, uint32_t width, uint32_t height, const float* in, float* out,
+                        uint8_t* rgba8);
+
+/* ---- asset ingest (host only: no context, no device work) -----------------
+ * Images: what stbi_load(path, &w, &h, &c, STBI_rgb_alpha) returns to
+ * Texture::GET (vk_textures.cpp:11-27) - RGBA8, row 0 first - for PNG (all
+ * colour types and depths, Adam7), JPEG (baseline and progressive, grey or
+ * YCbCr) and Radiance HDR (converted to 8 bits as stb does:
+ * pow(v, 1 / 2.2) * 255 + 0.5, clamped, truncated). The pixels are
+ * malloc'ed: release them with hr_image_free. err (may be NULL) receives a
+ * message on failure. */
+typedef struct hr_image {
+    uint8_t* rgba8;
+    uint32_t width, height;
+} hr_image;
+hr_status hr_image_load(const char* path, hr_image* out, char* err, size_t err_len);
+hr_status hr_image_decode(const uint8_t* data, size_t size, hr_image* out, char* err, size_t err_len);
+void hr_image_free(hr_image* image);
+
+/* Models: Prefab::GET(filename, invertNormals) (vk_mesh.cpp:843-901) for .obj
+ * (Mesh::load_from_obj, vk_mesh.cpp:96-150), .gltf and .glb
+ * (Prefab::loadNode / loadMaterial / loadTextures, vk_mesh.cpp:646-830).
+ * The view holds one mesh (the prefab's shared vertex / index buffer), its
+ * primitives in the reference's instance order (Node::node_to_instance,
+ * vk_mesh.cpp:494-515) with model = the node's global matrix (multiply by the
+ * entity matrix before hr_scene_build), the materials in first-use order
+ * (material index = position in this asset's list; the reference numbers
+ * them in one program-wide list) and the decoded textures the materials
+ * index. Image URIs are resolved against the model file's directory. The view
+ * stays valid until hr_asset_destroy. */
+typedef struct hr_asset hr_asset;
+#define HR_ASSET_INVERT_NORMALS 1u /* negate the root nodes' normals (the reference's invertNormals) */
+#define HR_ASSET_TEXTURE_SOURCE 2u /* glTF: image = textures[i].source (the reference uses images[i]) */
+typedef struct hr_asset_view {
+    hr_mesh mesh;
+    const hr_primitive* primitives;
+    uint32_t primitive_count;
+    const hr_material* materials;
+    uint32_t material_count;
+    const hr_texture* textures;
+    uint32_t texture_count;
+} hr_asset_view;
+hr_status hr_asset_load(const char* path, uint32_t flags, hr_asset** out, char* err, size_t err_len);
+hr_status hr_asset_view_get(const hr_asset* asset, hr_asset_view* out);
+void hr_asset_destroy(hr_asset* asset);
 
 /* ---- learned denoiser (no counterpart in the reference, whose only
  * "denoise" is the temporal shadow filter blur.comp) --------------------------

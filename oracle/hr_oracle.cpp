@@ -464,6 +464,9 @@ struct or_scene {
     uint32_t leaf_max = 4;
     uint32_t depth = 0;
     Tex sky, irr;
+    std::vector<uint32_t> texels;      /* material textures, RGBA8, red in the low byte */
+    std::vector<uint64_t> tex_offset;
+    std::vector<uint32_t> tex_w, tex_h;
 };
 
 namespace {
@@ -924,6 +927,50 @@ static inline int mat_index(const or_scene& s, float w) {
 }
 static inline v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 
+/* Material texture `id` at (u, v): NEAREST / REPEAT sampler
+ * (vk_initializers.cpp:305-316 with renderer.cpp:936, 3060), texel
+ * (floor(u w) mod w, floor(v h) mod h); R8G8B8A8_UNORM read as c / 255
+ * (vk_textures.cpp:24). */
+static inline v3 tex_rgba8(const or_scene& s, float id, float u, float v) {
+    int i = (int)id;
+    uint32_t w = s.tex_w[i], h = s.tex_h[i];
+    int x = wrap_index(floorf(u * (float)w), w);
+    int y = wrap_index(floorf(v * (float)h), h);
+    uint32_t c = s.texels[s.tex_offset[i] + (size_t)y * w + (size_t)x];
+    return mk((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f, (float)((c >> 16) & 0xFFu) / 255.0f);
+}
+static inline bool textured(const hr_material& m) {
+    return m.textures[0] > -1.0f || m.textures[1] > -1.0f || m.textures[2] > -1.0f || m.textures[3] > -1.0f;
+}
+/* Surface at a closest hit (hybridHit.rchit:47-68, closesthit.rchit:51-70):
+ * uv = (b0 uv0 + b1 uv1) + b2 uv2; albedo (before linearisation) = diffuse
+ * texel or the material colour (no vertex colour here, unlike the raster
+ * pass); emissive texel or 0; roughness / metallic = g / b of the
+ * metallic-roughness texel or the material's factors. */
+struct Surf {
+    v3 albedo, emissive;
+    float roughness, metallic;
+};
+static Surf hit_surface(const or_scene& s, const hr_material& m, const hr_vertex& x0, const hr_vertex& x1,
+                        const hr_vertex& x2, float b0, float b1, float b2) {
+    Surf f;
+    f.albedo = ld3(m.diffuse_ior);
+    f.emissive = mk(0.0f, 0.0f, 0.0f);
+    f.roughness = m.shading_metal_rough_idx[2];
+    f.metallic = m.shading_metal_rough_idx[1];
+    if (!textured(m)) return f;
+    float u = (x0.uv[0] * b0 + x1.uv[0] * b1) + x2.uv[0] * b2;
+    float v = (x0.uv[1] * b0 + x1.uv[1] * b1) + x2.uv[1] * b2;
+    if (m.textures[0] > -1.0f) f.albedo = tex_rgba8(s, m.textures[0], u, v);
+    if (m.textures[2] > -1.0f) f.emissive = tex_rgba8(s, m.textures[2], u, v);
+    if (m.textures[3] > -1.0f) {
+        v3 rm = tex_rgba8(s, m.textures[3], u, v);
+        f.roughness = rm.y;
+        f.metallic = rm.z;
+    }
+    return f;
+}
+
 /* direction of one shadow sample towards light i (helpers.glsl:82-93 for
  * disks, stratified cell s of S for rectangles) */
 static inline v3 light_sample_dir(const Frame& f, uint32_t i, v3 P, v3 L, Pcg& g, uint32_t s, uint32_t S) {
@@ -964,9 +1011,10 @@ static HitOut shade_hit(const or_scene& s, const Frame& f, v3 ro, v3 rd, const H
     v3 Lo = mk(0, 0, 0);
     const hr_material& mat = s.mats[p.material];
     int mode = (int)mat.shading_metal_rough_idx[0];
-    v3 albedo = pow3(ld3(mat.diffuse_ior), 2.2f);
-    float roughness = mat.shading_metal_rough_idx[2];
-    float metallic = mat.shading_metal_rough_idx[1];
+    const Surf sf = hit_surface(s, mat, x0, x1, x2, b0, b1, b2);
+    v3 albedo = pow3(sf.albedo, 2.2f);
+    float roughness = sf.roughness;
+    float metallic = sf.metallic;
     v3 F0 = mixs(mk(0.04f, 0.04f, 0.04f), albedo, metallic);
     v3 irradiance = env_lookup(s.irr, N, PI_H);
 
@@ -1054,7 +1102,7 @@ static HitOut shade_hit(const or_scene& s, const Frame& f, v3 ro, v3 rd, const H
     v3 F = FresnelSchlick(NdotV, F0);
     v3 kD = scl(sub(mk(1, 1, 1), F), 1.0f - metallic);
     v3 ambient = mul(mul(kD, albedo), irradiance);
-    out.color = add(Lo, ambient); /* + emissive (0: no textures) */
+    out.color = add(add(Lo, ambient), sf.emissive);
     out.dir = direction;
     out.scatter = scatter;
     out.origin = worldPos;
@@ -1087,9 +1135,10 @@ static HitOut shade_rt(const or_scene& s, const Frame& f, v3 ro, v3 rd, const Hi
     v3 Lo = mk(0, 0, 0);
     const hr_material& mat = s.mats[p.material];
     int mode = (int)mat.shading_metal_rough_idx[0];
-    v3 albedo = pow3(ld3(mat.diffuse_ior), 2.2f);
-    float roughness = mat.shading_metal_rough_idx[2];
-    float metallic = mat.shading_metal_rough_idx[1];
+    const Surf sf = hit_surface(s, mat, x0, x1, x2, b0, b1, b2);
+    v3 albedo = pow3(sf.albedo, 2.2f);
+    float roughness = sf.roughness;
+    float metallic = sf.metallic;
     v3 F0 = mixs(mk(0.04f, 0.04f, 0.04f), albedo, metallic);
     v3 irradiance = env_lookup(s.irr, N, PI_H);
     v3 direction = mk(1, 1, 1);
@@ -1178,7 +1227,7 @@ static HitOut shade_rt(const or_scene& s, const Frame& f, v3 ro, v3 rd, const Hi
     v3 F = FresnelSchlick(NdotV, F0);
     v3 kD = scl(sub(mk(1, 1, 1), F), 1.0f - metallic);
     v3 ambient = mul(mul(kD, albedo), irradiance);
-    out.color = add(Lo, ambient);
+    out.color = add(add(Lo, ambient), sf.emissive);
     out.dir = direction;
     out.scatter = scatter;
     out.origin = worldPos;
@@ -1301,6 +1350,23 @@ int or_scene_refit(or_scene* s, const float* models) {
     compute_depth(*s);
     return HR_OK;
 }
+int or_scene_set_textures(or_scene* s, const hr_texture* t, uint32_t n) {
+    if (!s || (n && !t)) return HR_E_INVALID;
+    s->texels.clear(); s->tex_offset.clear(); s->tex_w.clear(); s->tex_h.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!t[i].rgba8 || !t[i].width || !t[i].height) return HR_E_INVALID;
+        s->tex_offset.push_back(s->texels.size());
+        s->tex_w.push_back(t[i].width);
+        s->tex_h.push_back(t[i].height);
+        size_t m = (size_t)t[i].width * t[i].height;
+        for (size_t k = 0; k < m; ++k) {
+            const uint8_t* c = t[i].rgba8 + 4 * k;
+            s->texels.push_back((uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24));
+        }
+    }
+    return HR_OK;
+}
+
 int or_scene_set_env(or_scene* s, const hr_env* e) {
     if (!s || !e) return HR_E_INVALID;
     s->sky.w = e->sky_width; s->sky.h = e->sky_height;
@@ -1355,6 +1421,55 @@ void or_trace_rays(const or_scene* s, const hr_ray* rays, uint32_t n, int any_hi
     to_counters(total, cnt);
 }
 
+/* World-space direction of the primary ray through the centre of pixel
+ * (x, y) (raygen.rgen:29-40); x, y may lie outside the image. */
+static v3 primary_dir(const hr_camera* cam, uint32_t W, uint32_t H, uint32_t x, uint32_t y) {
+    float ux = ((float)x + 0.5f) / (float)W;
+    float uy = ((float)y + 0.5f) / (float)H;
+    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    float tgt[4], dir[4];
+    mul4(cam->inv_proj, dx, dy, 1.0f, 1.0f, tgt);
+    v3 tn = normalize(mk(tgt[0], tgt[1], tgt[2]));
+    mul4(cam->inv_view, tn.x, tn.y, tn.z, 0.0f, dir);
+    return mk(dir[0], dir[1], dir[2]);
+}
+/* The world position a rasteriser interpolates at pixel (x, y) for the
+ * triangle's plane (v0, ng): where the ray from o through that pixel meets it. */
+static v3 plane_point(const hr_camera* cam, uint32_t W, uint32_t H, v3 o, v3 v0, v3 ng, uint32_t x, uint32_t y) {
+    v3 d = primary_dir(cam, W, H, x, y);
+    float t = dot(sub(v0, o), ng) / dot(d, ng);
+    return add(o, scl(d, t));
+}
+/* perturbNormal + cotangent_frame (geometry_shader.frag:28-49). dFdx(p) and
+ * dFdy(p) are fine derivatives: differences across the pixel's 2x2 quad
+ * (columns x & ~1 and x | 1, rows y & ~1 and y | 1) of the positions on the
+ * hit triangle's plane. The shader's duv1 / duv2 are dFdx(p).xy / dFdy(p).xy
+ * (it differentiates p, not uv) and are kept so. Nin: interpolated, transformed,
+ * unnormalised normal; tx: normal-map texel. A zero or non-finite frame keeps
+ * normalize(Nin). */
+static v3 perturb_normal(const or_scene& s, const hr_camera* cam, uint32_t W, uint32_t H, v3 o, uint32_t tri,
+                         v3 Nin, v3 tx, uint32_t x, uint32_t y) {
+    v3 np = mk(tx.x * (255.0f / 127.0f) - 128.0f / 127.0f, tx.y * (255.0f / 127.0f) - 128.0f / 127.0f,
+               tx.z * (255.0f / 127.0f) - 128.0f / 127.0f);
+    const float* w = &s.wtri[(size_t)tri * 9];
+    v3 v0 = mk(w[0], w[1], w[2]), v1 = mk(w[3], w[4], w[5]), v2 = mk(w[6], w[7], w[8]);
+    v3 ng = cross(sub(v1, v0), sub(v2, v0));
+    uint32_t x0 = x & ~1u, x1 = x | 1u, y0 = y & ~1u, y1 = y | 1u;
+    v3 dp1 = sub(plane_point(cam, W, H, o, v0, ng, x1, y), plane_point(cam, W, H, o, v0, ng, x0, y));
+    v3 dp2 = sub(plane_point(cam, W, H, o, v0, ng, x, y1), plane_point(cam, W, H, o, v0, ng, x, y0));
+    v3 dp2perp = cross(dp2, Nin);
+    v3 dp1perp = cross(Nin, dp1);
+    v3 T = add(scl(dp2perp, dp1.x), scl(dp1perp, dp2.x));
+    v3 B = add(scl(dp2perp, dp1.y), scl(dp1perp, dp2.y));
+    float tt = dot(T, T), bb = dot(B, B);
+    float m = (bb > tt) ? bb : tt;
+    if (!(std::isfinite(tt) && std::isfinite(bb) && m > 0.0f)) return normalize(Nin);
+    float invmax = 1.0f / sqrtf(m);
+    T = scl(T, invmax);
+    B = scl(B, invmax);
+    return normalize(add(add(scl(T, np.x), scl(B, np.y)), scl(Nin, np.z)));
+}
+
 /* raygen.rgen:29-40 primary ray, then the G-buffer written by
  * geometry_shader.frag:51-76 (hits) and skybox.frag:16-21 (background). */
 void or_gbuffer_primary(const or_scene* s, const hr_camera* cam, uint32_t W, uint32_t H, float* position,
@@ -1364,15 +1479,9 @@ void or_gbuffer_primary(const or_scene* s, const hr_camera* cam, uint32_t W, uin
     parallel_rows(H, row_start, row_step, threads, [&](uint32_t y, Cnt& c) {
         for (uint32_t x = 0; x < W; ++x) {
             size_t p = ((size_t)y * W + x) * 4;
-            float ux = ((float)x + 0.5f) / (float)W;
-            float uy = ((float)y + 0.5f) / (float)H;
-            float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
-            float org[4], tgt[4], dir[4];
+            float org[4];
             mul4(cam->inv_view, 0.0f, 0.0f, 0.0f, 1.0f, org);
-            mul4(cam->inv_proj, dx, dy, 1.0f, 1.0f, tgt);
-            v3 tn = normalize(mk(tgt[0], tgt[1], tgt[2]));
-            mul4(cam->inv_view, tn.x, tn.y, tn.z, 0.0f, dir);
-            v3 o = mk(org[0], org[1], org[2]), d = mk(dir[0], dir[1], dir[2]);
+            v3 o = mk(org[0], org[1], org[2]), d = primary_dir(cam, W, H, x, y);
             v3 ro = add(o, scl(d, 1e-2f));
             RayP r = prep_ray(ro, d, 0.001f, 10000.0f);
             Hit h;
@@ -1395,15 +1504,32 @@ void or_gbuffer_primary(const or_scene* s, const hr_camera* cam, uint32_t W, uin
             float b1 = h.u, b2 = h.v, b0 = 1.0f - b1 - b2;
             v3 nrm = add(add(scl(ld3(x0.normal), b0), scl(ld3(x1.normal), b1)), scl(ld3(x2.normal), b2));
             const float* M = pr.nm;
-            v3 N = normalize(mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z,
-                                (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
-                                (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z));
+            /* inNormal of basic.vert:44, interpolated, not yet normalised */
+            v3 Nin = mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z, (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
+                        (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z);
+            v3 N = normalize(Nin);
             v3 wp = add(ro, scl(d, h.t));
             const hr_material& mt = s->mats[pr.material];
+            v3 alb = ld3(mt.diffuse_ior);
+            v3 emi = mk(0.0f, 0.0f, 0.0f);
+            v3 mtl = mk(0.0f, mt.shading_metal_rough_idx[2], mt.shading_metal_rough_idx[1]);
+            if (textured(mt)) { /* geometry_shader.frag:54-64 */
+                float u = (x0.uv[0] * b0 + x1.uv[0] * b1) + x2.uv[0] * b2;
+                float v = (x0.uv[1] * b0 + x1.uv[1] * b1) + x2.uv[1] * b2;
+                if (mt.textures[0] > -1.0f) {
+                    v3 col = add(add(scl(ld3(x0.color), b0), scl(ld3(x1.color), b1)), scl(ld3(x2.color), b2));
+                    alb = mul(tex_rgba8(*s, mt.textures[0], u, v), col);
+                }
+                if (mt.textures[2] > -1.0f) emi = tex_rgba8(*s, mt.textures[2], u, v);
+                if (mt.textures[3] > -1.0f) mtl = tex_rgba8(*s, mt.textures[3], u, v);
+                if (mt.textures[1] > -1.0f)
+                    N = perturb_normal(*s, cam, W, H, o, h.tri, Nin, tex_rgba8(*s, mt.textures[1], u, v), x, y);
+            }
             P[0] = wp.x; P[1] = wp.y; P[2] = wp.z; P[3] = mt.shading_metal_rough_idx[3];
             Nn[0] = N.x; Nn[1] = N.y; Nn[2] = N.z; Nn[3] = 1.0f;
-            A[0] = mt.diffuse_ior[0]; A[1] = mt.diffuse_ior[1]; A[2] = mt.diffuse_ior[2]; A[3] = 1.0f;
-            Mt[0] = 0.0f; Mt[1] = mt.shading_metal_rough_idx[2]; Mt[2] = mt.shading_metal_rough_idx[1]; Mt[3] = 0.0f;
+            A[0] = alb.x; A[1] = alb.y; A[2] = alb.z; A[3] = 1.0f;
+            Mt[0] = mtl.x; Mt[1] = mtl.y; Mt[2] = mtl.z; Mt[3] = 0.0f;
+            E[0] = emi.x; E[1] = emi.y; E[2] = emi.z; E[3] = 0.0f;
         }
     }, total);
     to_counters(total, cnt);
@@ -1704,3 +1830,17 @@ void or_render_rt(const or_scene* s, const hr_light* lights, const hr_light_shap
     to_counters(total, cnt);
 }
 } /* extern "C" */
+
+/* postFragment.frag:11-20 over n float4 pixels: rgb = pow(max(rgb, 0.001),
+ * 1 / 2.2), alpha kept; rgba8 (may be NULL) = rint(clamp(v, 0, 1) * 255). */
+void or_post_gamma(const float* in, uint64_t n, float* out, uint8_t* rgba8) {
+    const float g = 1.0f / 2.2f;
+    for (uint64_t i = 0; i < n; ++i) {
+        float o[4] = {powf(fmaxf(in[4 * i], 0.001f), g), powf(fmaxf(in[4 * i + 1], 0.001f), g),
+                      powf(fmaxf(in[4 * i + 2], 0.001f), g), in[4 * i + 3]};
+        for (int k = 0; k < 4; ++k) {
+            if (out) out[4 * i + k] = o[k];
+            if (rgba8) rgba8[4 * i + k] = (uint8_t)rintf(fminf(fmaxf(o[k], 0.0f), 1.0f) * 255.0f);
+        }
+    }
+}

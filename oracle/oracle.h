@@ -58,6 +58,8 @@ int or_scene_build(const hr_mesh* meshes, uint32_t mesh_count,
                    uint32_t leaf_max, or_scene** out);
 int or_scene_refit(or_scene* s, const float* models);
 int or_scene_set_env(or_scene* s, const hr_env* env);
+/* material textures, as hr_scene_set_textures */
+int or_scene_set_textures(or_scene* s, const hr_texture* textures, uint32_t count);
 void or_scene_destroy(or_scene* s);
 uint32_t or_scene_tri_count(const or_scene* s);
 uint32_t or_scene_node_count(const or_scene* s);
@@ -88,6 +90,8 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H,
                      const hr_camera* cam, const hr_trace_params* params, const float* shadow,
                      float* lit, float* refl, uint32_t row_start, uint32_t row_step,
                      or_counters* cnt, uint32_t threads);
+/* display pass (postFragment.frag) over n float4 pixels; out / rgba8 may be NULL */
+void or_post_gamma(const float* in, uint64_t n, float* out, uint8_t* rgba8);
 /* pure ray tracing mode (raygen.rgen, closesthit.rchit): accum float4[H][W] */
 void or_render_rt(const or_scene* s, const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
                   const hr_camera* cam, const hr_trace_params* params, uint32_t W, uint32_t H, float* accum,

@@ -68,6 +68,9 @@ def lib():
         L.or_scene_refit.argtypes = [vp, vp]
         L.or_scene_set_env.restype = i32
         L.or_scene_set_env.argtypes = [vp, C.POINTER(abi.hr_env)]
+        L.or_scene_set_textures.restype = i32
+        L.or_scene_set_textures.argtypes = [vp, C.POINTER(abi.hr_texture), u32]
+        L.or_post_gamma.argtypes = [vp, C.c_uint64, vp, vp]
         L.or_scene_destroy.argtypes = [vp]
         for n in ("or_scene_tri_count", "or_scene_node_count", "or_scene_depth"):
             getattr(L, n).restype = u32
@@ -110,6 +113,10 @@ class OracleScene:
             raise RuntimeError("or_scene_build failed: %d" % rc)
         self.h = h
         L.or_scene_set_env(h, C.byref(self.packed.env))
+        if self.packed.texture_count:
+            rc = L.or_scene_set_textures(h, self.packed.textures, self.packed.texture_count)
+            if rc != 0:
+                raise RuntimeError("or_scene_set_textures failed: %d" % rc)
 
     def close(self):
         if self.h:
@@ -313,3 +320,13 @@ def intersect_tri(org, d, tmin, tmax, v0, v1, v2):
     ok = lib().or_intersect_tri(org.ctypes.data, d.ctypes.data, tmin, tmax, v0.ctypes.data, v1.ctypes.data,
                                 v2.ctypes.data, C.byref(t), C.byref(u), C.byref(v))
     return (t.value, u.value, v.value) if ok else None
+
+
+def post_gamma(img):
+    """postFragment.frag over a float32 (..., 4) image: (float result, RGBA8)."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    n = a.size // 4
+    out = np.empty_like(a)
+    q = np.empty(a.shape, dtype=np.uint8)
+    lib().or_post_gamma(a.ctypes.data, n, out.ctypes.data, q.ctypes.data)
+    return out, q

@@ -58,6 +58,20 @@ class hr_env(C.Structure):
                 ("irradiance", C.c_void_p), ("irr_width", C.c_uint32), ("irr_height", C.c_uint32)]
 
 
+class hr_texture(C.Structure):
+    _fields_ = [("rgba8", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class hr_image(C.Structure):
+    _fields_ = [("rgba8", C.POINTER(C.c_uint8)), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class hr_asset_view(C.Structure):
+    _fields_ = [("mesh", hr_mesh), ("primitives", C.POINTER(hr_primitive)), ("primitive_count", C.c_uint32),
+                ("materials", C.POINTER(hr_material)), ("material_count", C.c_uint32),
+                ("textures", C.POINTER(hr_texture)), ("texture_count", C.c_uint32)]
+
+
 class hr_gbuffer(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("position", C.c_void_p),
                 ("normal", C.c_void_p), ("albedo", C.c_void_p), ("material", C.c_void_p),
@@ -172,6 +186,14 @@ class PackedScene:
         irr = np.ascontiguousarray(scene.env_irr, dtype=np.float32)
         self._keep += [sky, irr]
         self.env = hr_env(ptr(sky), sky.shape[1], sky.shape[0], ptr(irr), irr.shape[1], irr.shape[0])
+        texs = list(getattr(scene, "textures", None) or [])
+        self.texture_count = len(texs)
+        self.textures = (hr_texture * max(1, len(texs)))()
+        for i, t in enumerate(texs):
+            t = np.ascontiguousarray(t, dtype=np.uint8)
+            assert t.ndim == 3 and t.shape[2] == 4, "textures are (height, width, 4) uint8"
+            self._keep.append(t)
+            self.textures[i] = hr_texture(ptr(t), t.shape[1], t.shape[0])
 
 
 def pack_lights(lights, shapes=None):

@@ -10,8 +10,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libhr.so")
-SOURCES = ["csrc/lbvh.hip", "csrc/trace.hip", "csrc/wavefront.hip", "csrc/denoise.hip", "csrc/temporal.hip", "csrc/gbuffer_io.hip", "csrc/rt.hip", "csrc/hr_api.hip", "csrc/dn_api.hip"]
-HEADERS = ["csrc/device_math.h", "csrc/internal.h", "csrc/trace_common.h", "csrc/api_internal.h"]
+SOURCES = ["csrc/lbvh.hip", "csrc/trace.hip", "csrc/wavefront.hip", "csrc/denoise.hip", "csrc/temporal.hip",
+           "csrc/gbuffer_io.hip", "csrc/rt.hip", "csrc/hr_api.hip", "csrc/dn_api.hip",
+           "csrc/image_io.cpp", "csrc/assets.cpp"]  # the .cpp files are host-only (asset ingest)
+HEADERS = ["csrc/device_math.h", "csrc/internal.h", "csrc/trace_common.h", "csrc/api_internal.h", "csrc/image_io.h"]
+LIBS = ["-lz"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
@@ -26,7 +29,7 @@ def hipcc():
 
 def build(out=LIB, extra=(), verbose=False):
     cmd = [hipcc()] + FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include")] + \
-        [os.path.join(HERE, s) for s in SOURCES] + ["-o", out + ".tmp"]
+        [os.path.join(HERE, s) for s in SOURCES] + LIBS + ["-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd))
     subprocess.check_call(cmd)

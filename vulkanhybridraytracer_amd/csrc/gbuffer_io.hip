@@ -9,6 +9,7 @@
 //               writes them (normal * 0.5 + 0.5, material (0, rough, metal))
 //               and the attachments store them: half by round to nearest
 //               even, UNORM8 = rint(clamp(v, 0, 1) * 255)
+//   k_post_gamma  the display pass of postFragment.frag
 // One lane per pixel; every plane is read or written as one 8-byte (RGBA16F),
 // 4-byte (RGBA8, RG16F) or 16-byte (FP32) access, so a wave's accesses are
 // contiguous.
@@ -73,7 +74,25 @@ __global__ __launch_bounds__(256) void k_quantize(uint32_t n, GBufView G, const 
     }
 }
 
+// postFragment.frag:11-20: rgb = pow(max(rgb, 0.001), 1 / 2.2), alpha kept;
+// optionally stored to an RGBA8 UNORM image
+__global__ __launch_bounds__(256) void k_post_gamma(uint32_t n, const float4* in, float4* out, uint32_t* rgba8) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = in[i];
+    const float g = 1.0f / 2.2f;
+    const float4 o = make_float4(powf(fmaxf(c.x, 0.001f), g), powf(fmaxf(c.y, 0.001f), g),
+                                 powf(fmaxf(c.z, 0.001f), g), c.w);
+    if (out) out[i] = o;
+    if (rgba8) rgba8[i] = pack8(o.x, o.y, o.z, o.w);
+}
+
 }  // namespace
+
+hipError_t launch_post_gamma(uint32_t n, const float4* in, float4* out, uint32_t* rgba8, hipStream_t st) {
+    k_post_gamma<<<(n + 255) / 256, 256, 0, st>>>(n, in, out, rgba8);
+    return hipGetLastError();
+}
 
 hipError_t launch_gbuffer_ingest(uint32_t n, const void* pos, const void* nrm, const void* alb, const void* mot,
                                  const void* mat, const void* emi, GBufView G, float* motion_out, hipStream_t st) {

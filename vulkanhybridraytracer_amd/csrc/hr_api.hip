@@ -32,6 +32,10 @@ struct hr_scene {
     float4* d_sky = nullptr;
     float4* d_irr = nullptr;
     uint32_t sky_w = 0, sky_h = 0, irr_w = 0, irr_h = 0;
+    uint32_t* d_texels = nullptr;  // owned here, replaced by hr_scene_set_textures
+    TexInfo* d_tex = nullptr;
+    uint32_t ntex = 0;
+    int64_t max_tex = -1;          // largest texture index the materials use
     BuildBuffers bb{};
 };
 
@@ -78,6 +82,11 @@ DevScene dev_scene(const hr_scene* s) {
     d.prims = s->d_prims;
     d.indices = s->d_indices;
     d.vnormal = s->d_vnormal;
+    d.verts = s->d_verts;
+    d.wtri = s->bb.wtri;
+    d.texels = s->d_texels;
+    d.tex = s->d_tex;
+    d.ntex = s->ntex;
     d.mats = s->d_mats;
     d.sky = s->d_sky;
     d.irr = s->d_irr;
@@ -97,9 +106,12 @@ hr_status check_material(hr_context* ctx, const hr_material& m, uint32_t i, uint
     if ((float)mode != m.shading_metal_rough_idx[0] || (mode != 0 && mode != 3 && mode != 4))
         return fail(ctx, HR_E_INVALID, "material %u: shading model %g is not 0, 3 or 4", i,
                     m.shading_metal_rough_idx[0]);
-    for (int k = 0; k < 4; ++k)
-        if (m.textures[k] > -1.0f)
-            return fail(ctx, HR_E_UNSUPPORTED, "material %u: textured materials are not supported", i);
+    // a texture slot is used when its value is above -1 (hybridHit.rchit:63-65)
+    for (int k = 0; k < 4; ++k) {
+        float t = m.textures[k];
+        if (t > -1.0f && !(t < 16777216.0f && (float)(int)t == t))
+            return fail(ctx, HR_E_INVALID, "material %u: texture index %g is not a whole number", i, t);
+    }
     float w = m.shading_metal_rough_idx[3];
     if (!(w >= 0.0f) || w >= (float)nmat || (float)(int)w != w)
         return fail(ctx, HR_E_INVALID, "material %u: material index %g out of range", i, w);
@@ -107,6 +119,21 @@ hr_status check_material(hr_context* ctx, const hr_material& m, uint32_t i, uint
         const float* f = &m.diffuse_ior[0];
         if (!std::isfinite(f[k])) return fail(ctx, HR_E_INVALID, "material %u: non-finite value", i);
     }
+    return HR_OK;
+}
+
+int64_t max_texture(const hr_material* m, uint32_t n) {
+    int64_t mx = -1;
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (m[i].textures[k] > -1.0f && (int64_t)m[i].textures[k] > mx) mx = (int64_t)m[i].textures[k];
+    return mx;
+}
+// every texture a material names must be set before a pass that shades
+hr_status check_textures(hr_context* ctx, const hr_scene* s) {
+    if (s->max_tex >= (int64_t)s->ntex)
+        return fail(ctx, HR_E_INVALID, "materials use texture %lld but %u textures are set",
+                    (long long)s->max_tex, s->ntex);
     return HR_OK;
 }
 
@@ -293,6 +320,7 @@ hr_status plan_frame(hr_context* ctx, const hr_frame_desc& d, FramePlan& fp, boo
     HR_TRY(fill_frame(ctx, fp.f, &d.camera, d.gbuffer.width, d.gbuffer.height));
     HR_TRY(fill_lights(ctx, fp.f, d.lights, d.shapes, d.light_count, &d.params));
     if (need_shading && d.params.refl_spp > 64) return fail(ctx, HR_E_UNSUPPORTED, "refl_spp above 64");
+    if (need_shading) HR_TRY(check_textures(ctx, d.scene));
     fp.scene = d.scene;
     fp.g = gview(&d.gbuffer);
     fp.shadow = d.shadow;
@@ -421,6 +449,8 @@ void hr_scene_destroy(hr_scene* s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     for (void* p : s->allocs) (void)hipFree(p);
+    if (s->d_texels) (void)hipFree(s->d_texels);
+    if (s->d_tex) (void)hipFree(s->d_tex);
     delete s;
 }
 
@@ -458,6 +488,7 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     s->mesh_count = mesh_count;
     s->prim_count = prim_count;
     s->nmat = material_count;
+    s->max_tex = max_texture(materials, material_count);
     s->leaf_max = leaf_max;
     uint64_t tri = 0;
     s->h_prims.resize(prim_count);
@@ -650,6 +681,46 @@ hr_status hr_scene_set_materials(hr_scene* s, const hr_material* materials, uint
     HR_HIP(ctx, hipSetDevice(ctx->device));
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     HR_HIP(ctx, hipMemcpy(s->d_mats, materials, count * sizeof(hr_material), hipMemcpyHostToDevice));
+    s->max_tex = max_texture(materials, count);
+    return HR_OK;
+}
+
+hr_status hr_scene_set_textures(hr_scene* s, const hr_texture* textures, uint32_t count) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (count && !textures) return fail(ctx, HR_E_INVALID, "textures is NULL");
+    std::vector<TexInfo> info(count);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const hr_texture& t = textures[i];
+        if (!t.rgba8 || t.width == 0 || t.height == 0 || t.width > 65536 || t.height > 65536)
+            return fail(ctx, HR_E_INVALID, "texture %u: empty, NULL or larger than 65536 texels a side", i);
+        info[i].offset = total;
+        info[i].w = t.width;
+        info[i].h = t.height;
+        total += (uint64_t)t.width * t.height;
+    }
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t* texels = nullptr;
+    TexInfo* tex = nullptr;
+    hipError_t e = hipMalloc(&texels, (total ? total : 1) * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc textures");
+    e = hipMalloc(&tex, (count ? count : 1) * sizeof(TexInfo));
+    for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+        e = hipMemcpy(texels + info[i].offset, textures[i].rgba8, (size_t)info[i].w * info[i].h * 4,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && count) e = hipMemcpy(tex, info.data(), count * sizeof(TexInfo), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(texels);
+        if (tex) (void)hipFree(tex);
+        return hip_fail(ctx, e, "texture upload");
+    }
+    if (s->d_texels) (void)hipFree(s->d_texels);
+    if (s->d_tex) (void)hipFree(s->d_tex);
+    s->d_texels = texels;
+    s->d_tex = tex;
+    s->ntex = count;
     return HR_OK;
 }
 
@@ -731,6 +802,7 @@ hr_status hr_gbuffer_primary(hr_scene* s, const hr_camera* cam, const hr_gbuffer
     HR_TRY(check_gbuffer(ctx, gb));
     FrameArgs f;
     HR_TRY(fill_frame(ctx, f, cam, gb->width, gb->height));
+    HR_TRY(check_textures(ctx, s));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, launch_gbuffer(dev_scene(s), f, gview(gb), ctx->d_counters, false, ctx->stream));
     return frame_end(ctx, stats, stats ? &stats->ms_gbuffer : nullptr);
@@ -749,6 +821,7 @@ hr_status hr_render_rt(hr_scene* s, const hr_light* lights, const hr_light_shape
     HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
     if (f.max_depth == 0) return fail(ctx, HR_E_INVALID, "max_depth must be at least 1");
     if (f.adaptive) return fail(ctx, HR_E_UNSUPPORTED, "adaptive sampling is a hybrid-pass option");
+    HR_TRY(check_textures(ctx, s));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, launch_rt(dev_scene(s), f, reinterpret_cast<float4*>(accum), (int32_t)(int)cam->frame[0],
                           ctx->d_counters, params->flags & HR_FLAG_COUNT_VISITS, ctx->stream));
@@ -790,6 +863,7 @@ hr_status hr_shade_hybrid(hr_scene* s, const hr_gbuffer* gb, const hr_light* lig
     HR_TRY(fill_lights(ctx, f, lights, shapes, nl, params));
     if (params->refl_spp > 64) return fail(ctx, HR_E_UNSUPPORTED, "refl_spp above 64");
     bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    HR_TRY(check_textures(ctx, s));
     WaveArgs w;
     HR_TRY(wave_args(ctx, f, w));
     HR_TRY(frame_begin(ctx, stats != nullptr));
@@ -1045,6 +1119,18 @@ hr_status hr_motion_vectors(hr_context* ctx, const hr_gbuffer* gb, const float* 
     HR_HIP(ctx, hipSetDevice(ctx->device));
     HR_HIP(ctx, launch_motion(gb->width, gb->height, reinterpret_cast<const float4*>(gb->position),
                               reinterpret_cast<const float4*>(gb->normal), a, motion, ctx->stream));
+    return HR_OK;
+}
+
+hr_status hr_post_gamma(hr_context* ctx, uint32_t width, uint32_t height, const float* in, float* out,
+                        uint8_t* rgba8) {
+    if (!ctx) return HR_E_INVALID;
+    if (!in || (!out && !rgba8)) return fail(ctx, HR_E_INVALID, "in and at least one output are required");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(ctx, HR_E_INVALID, "image size %ux%u out of range", width, height);
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, launch_post_gamma(width * height, reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out),
+                                  reinterpret_cast<uint32_t*>(rgba8), ctx->stream));
     return HR_OK;
 }
 

@@ -45,6 +45,14 @@ static_assert(sizeof(TNode) == 64, "TNode");
 static_assert(sizeof(STri) == 48, "STri");
 static_assert(sizeof(PrimInfo) == 64, "PrimInfo");
 
+// One material texture inside the scene's texel array (RGBA8 packed in a
+// uint32, red in the low byte).
+struct TexInfo {
+    uint64_t offset;  // first texel
+    uint32_t w, h;
+};
+static_assert(sizeof(TexInfo) == 16, "TexInfo");
+
 // Read-only view of a built scene, passed to kernels by value.
 struct DevScene {
     const TNode* nodes;
@@ -53,11 +61,16 @@ struct DevScene {
     const PrimInfo* prims;
     const uint32_t* indices;    // concatenated mesh indices
     const float4* vnormal;      // concatenated per-vertex normals (xyz, 0)
+    const uint8_t* verts;       // concatenated hr_vertex records (44 B: colour at 24, uv at 36)
+    const float4* wtri;         // world-space triangle vertices, 3 per triangle id
     const hr_material* mats;
     const float4* sky;
     const float4* irr;
+    const uint32_t* texels;     // material textures, RGBA8
+    const TexInfo* tex;
     uint32_t node_count, tri_count, nmat, prim_count;
     uint32_t sky_w, sky_h, irr_w, irr_h;
+    uint32_t ntex;
 };
 
 // Per-frame uniforms, passed to kernels by value (replaces the camera, light,
@@ -159,6 +172,8 @@ hipError_t launch_gbuffer_ingest(uint32_t n, const void* pos, const void* nrm, c
                                  const void* mat, const void* emi, GBufView G, float* motion_out, hipStream_t st);
 hipError_t launch_gbuffer_quantize(uint32_t n, GBufView G, const float* motion_in, void* pos, void* nrm, void* alb,
                                    void* mot, void* mat, void* emi, hipStream_t st);
+// post pass (postFragment.frag): float4 in -> float4 out and/or RGBA8
+hipError_t launch_post_gamma(uint32_t n, const float4* in, float4* out, uint32_t* rgba8, hipStream_t st);
 
 // ---- LBVH build (lbvh.hip) ------------------------------------------------------
 struct BuildBuffers {

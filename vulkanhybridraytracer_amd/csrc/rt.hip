@@ -32,9 +32,9 @@ __device__ RtOut shade_rt(const DevScene& S, const FrameArgs& F, v3 rd, const Hi
     const float NdotV = g.NdotV;
     v3 Lo = mk(0.0f, 0.0f, 0.0f);
     const int mode = (int)mat.shading_metal_rough_idx[0];
-    const v3 albedo = pow3(ld3p(mat.diffuse_ior), 2.2f);
-    const float roughness = mat.shading_metal_rough_idx[2];
-    const float metallic = mat.shading_metal_rough_idx[1];
+    const v3 albedo = pow3(g.albedo, 2.2f);
+    const float roughness = g.roughness;
+    const float metallic = g.metallic;
     const v3 F0 = mixs(mk(0.04f, 0.04f, 0.04f), albedo, metallic);
     const v3 irradiance = env_lookup(S.irr, S.irr_w, S.irr_h, N, PI_H);
     v3 direction = mk(1.0f, 1.0f, 1.0f);
@@ -123,7 +123,7 @@ __device__ RtOut shade_rt(const DevScene& S, const FrameArgs& F, v3 rd, const Hi
     const v3 Fa = FresnelSchlick(NdotV, F0);
     const v3 kD = scl(sub(mk(1.0f, 1.0f, 1.0f), Fa), 1.0f - metallic);
     const v3 ambient = mul(mul(kD, albedo), irradiance);
-    out.color = add(Lo, ambient);
+    out.color = add(add(Lo, ambient), g.emissive);
     out.dir = direction;
     out.scatter = scatter;
     return out;

@@ -24,6 +24,55 @@ using namespace hrd;
 using namespace tc;
 
 namespace {
+// World-space direction of the primary ray through the centre of pixel (x, y)
+// (raygen.rgen:29-40); x and y may lie outside the image.
+__device__ __forceinline__ v3 primary_dir(const FrameArgs& F, uint32_t x, uint32_t y) {
+    float ux = ((float)x + 0.5f) / (float)F.width;
+    float uy = ((float)y + 0.5f) / (float)F.height;
+    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    float tgt[4], dir[4];
+    mul4(F.inv_proj, dx, dy, 1.0f, 1.0f, tgt);
+    v3 tn = normalize(mk(tgt[0], tgt[1], tgt[2]));
+    mul4(F.inv_view, tn.x, tn.y, tn.z, 0.0f, dir);
+    return mk(dir[0], dir[1], dir[2]);
+}
+// Point where the ray from o through pixel (x, y) meets the plane of the
+// triangle (v0, ng): the world position a rasteriser interpolates for that
+// pixel, helper pixels included.
+__device__ __forceinline__ v3 plane_point(const FrameArgs& F, v3 o, v3 v0, v3 ng, uint32_t x, uint32_t y) {
+    v3 d = primary_dir(F, x, y);
+    float t = dot(sub(v0, o), ng) / dot(d, ng);
+    return add(o, scl(d, t));
+}
+// perturbNormal / cotangent_frame of geometry_shader.frag:28-49 for pixel
+// (x, y): the screen-space derivatives dFdx(p), dFdy(p) are the fine
+// differences across the pixel's 2x2 quad of the world positions on the hit
+// triangle's plane; the frame uses dFdx(p).xy and dFdy(p).xy as the uv
+// derivatives, as the shader does. Nin is the interpolated, transformed,
+// unnormalised vertex normal; tx the normal-map texel. A degenerate frame
+// (zero or non-finite) keeps normalize(Nin).
+__device__ v3 perturb_normal(const FrameArgs& F, v3 o, const float4* wtri, uint32_t tri, v3 Nin, v3 tx,
+                             uint32_t x, uint32_t y) {
+    v3 np = mk(tx.x * (255.0f / 127.0f) - 128.0f / 127.0f, tx.y * (255.0f / 127.0f) - 128.0f / 127.0f,
+               tx.z * (255.0f / 127.0f) - 128.0f / 127.0f);
+    v3 v0 = ld3(wtri[3 * (size_t)tri]), v1 = ld3(wtri[3 * (size_t)tri + 1]), v2 = ld3(wtri[3 * (size_t)tri + 2]);
+    v3 ng = cross(sub(v1, v0), sub(v2, v0));
+    uint32_t x0 = x & ~1u, x1 = x | 1u, y0 = y & ~1u, y1 = y | 1u;
+    v3 dp1 = sub(plane_point(F, o, v0, ng, x1, y), plane_point(F, o, v0, ng, x0, y));
+    v3 dp2 = sub(plane_point(F, o, v0, ng, x, y1), plane_point(F, o, v0, ng, x, y0));
+    v3 dp2perp = cross(dp2, Nin);
+    v3 dp1perp = cross(Nin, dp1);
+    v3 T = add(scl(dp2perp, dp1.x), scl(dp1perp, dp2.x));
+    v3 B = add(scl(dp2perp, dp1.y), scl(dp1perp, dp2.y));
+    float tt = dot(T, T), bb = dot(B, B);
+    float m = (bb > tt) ? bb : tt;
+    if (!(isfinite(tt) && isfinite(bb) && m > 0.0f)) return normalize(Nin);
+    float invmax = 1.0f / sqrtf(m);
+    T = scl(T, invmax);
+    B = scl(B, invmax);
+    return normalize(add(add(scl(T, np.x), scl(B, np.y)), scl(Nin, np.z)));
+}
+
 // ---- kernels ------------------------------------------------------------------
 template <bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBufView G,
@@ -35,15 +84,9 @@ __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBuf
     pixel_of(F.width, x, y);
     if (x < F.width && y < F.height) {
         size_t p = (size_t)y * F.width + x;
-        float ux = ((float)x + 0.5f) / (float)F.width;
-        float uy = ((float)y + 0.5f) / (float)F.height;
-        float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
-        float org[4], tgt[4], dir[4];
+        float org[4];
         mul4(F.inv_view, 0.0f, 0.0f, 0.0f, 1.0f, org);
-        mul4(F.inv_proj, dx, dy, 1.0f, 1.0f, tgt);
-        v3 tn = normalize(mk(tgt[0], tgt[1], tgt[2]));
-        mul4(F.inv_view, tn.x, tn.y, tn.z, 0.0f, dir);
-        v3 o = mk(org[0], org[1], org[2]), d = mk(dir[0], dir[1], dir[2]);
+        v3 o = mk(org[0], org[1], org[2]), d = primary_dir(F, x, y);
         v3 ro = add(o, scl(d, 1e-2f));
         RayP r = prep_ray(ro, d);
         HitD h;
@@ -57,21 +100,43 @@ __global__ __launch_bounds__(BLOCK) void k_gbuffer(DevScene S, FrameArgs F, GBuf
             const PrimInfo& pr = S.prims[S.tri_prim[h.tri]];
             uint32_t local = h.tri - pr.tri_offset;
             uint32_t ib = pr.index_base + 3 * local;
-            v3 n0 = ld3(S.vnormal[pr.vertex_base + S.indices[ib + 0]]);
-            v3 n1 = ld3(S.vnormal[pr.vertex_base + S.indices[ib + 1]]);
-            v3 n2 = ld3(S.vnormal[pr.vertex_base + S.indices[ib + 2]]);
+            uint32_t vi[3] = {pr.vertex_base + S.indices[ib + 0], pr.vertex_base + S.indices[ib + 1],
+                              pr.vertex_base + S.indices[ib + 2]};
+            v3 n0 = ld3(S.vnormal[vi[0]]);
+            v3 n1 = ld3(S.vnormal[vi[1]]);
+            v3 n2 = ld3(S.vnormal[vi[2]]);
             float b1 = h.u, b2 = h.v, b0 = 1.0f - b1 - b2;
             v3 nrm = add(add(scl(n0, b0), scl(n1, b1)), scl(n2, b2));
             const float* M = pr.nm;
-            v3 N = normalize(mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z,
-                                (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
-                                (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z));
+            v3 Nin = mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z, (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
+                        (M[6] * nrm.x + M[7] * nrm.y) + M[8] * nrm.z);
+            v3 N = normalize(Nin);
             v3 wp = add(ro, scl(d, h.t));
             const hr_material& mt = S.mats[pr.material];
+            v3 alb = ld3p(mt.diffuse_ior);
+            v3 emi = mk(0.0f, 0.0f, 0.0f);
+            v3 mtl = mk(0.0f, mt.shading_metal_rough_idx[2], mt.shading_metal_rough_idx[1]);
+            if (textured(mt)) {  // geometry_shader.frag:54-64
+                float u, v;
+                interp_uv(S, vi, b0, b1, b2, u, v);
+                if (mt.textures[0] > -1.0f) {
+                    const float* a = vtx(S, vi[0]);
+                    const float* b = vtx(S, vi[1]);
+                    const float* e = vtx(S, vi[2]);
+                    v3 col = add(add(scl(mk(a[6], a[7], a[8]), b0), scl(mk(b[6], b[7], b[8]), b1)),
+                                 scl(mk(e[6], e[7], e[8]), b2));
+                    alb = mul(tex_rgba8(S, mt.textures[0], u, v), col);
+                }
+                if (mt.textures[2] > -1.0f) emi = tex_rgba8(S, mt.textures[2], u, v);
+                if (mt.textures[3] > -1.0f) mtl = tex_rgba8(S, mt.textures[3], u, v);
+                if (mt.textures[1] > -1.0f)
+                    N = perturb_normal(F, o, S.wtri, h.tri, Nin, tex_rgba8(S, mt.textures[1], u, v), x, y);
+            }
             P = make_float4(wp.x, wp.y, wp.z, mt.shading_metal_rough_idx[3]);
             Nn = make_float4(N.x, N.y, N.z, 1.0f);
-            A = make_float4(mt.diffuse_ior[0], mt.diffuse_ior[1], mt.diffuse_ior[2], 1.0f);
-            Mt = make_float4(0.0f, mt.shading_metal_rough_idx[2], mt.shading_metal_rough_idx[1], 0.0f);
+            A = make_float4(alb.x, alb.y, alb.z, 1.0f);
+            Mt = make_float4(mtl.x, mtl.y, mtl.z, 0.0f);
+            E = make_float4(emi.x, emi.y, emi.z, 0.0f);
         }
         G.pos[p] = P;
         G.nrm[p] = Nn;

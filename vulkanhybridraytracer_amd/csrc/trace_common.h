@@ -429,11 +429,43 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
     }
 }
 
-// Geometry of a closest hit (hybridHit.rchit:26-53, closesthit.rchit:26-54).
+// ---- material textures ----------------------------------------------------------
+// Texel of material texture `id` (a whole number >= 0) at (u, v): NEAREST /
+// REPEAT, R8G8B8A8_UNORM decoded as c / 255.
+__device__ __forceinline__ v3 tex_rgba8(const DevScene& S, float id, float u, float v) {
+    const TexInfo t = S.tex[(int)id];
+    int x = wrap_index(floorf(u * (float)t.w), t.w);
+    int y = wrap_index(floorf(v * (float)t.h), t.h);
+    uint32_t c = __ldg(S.texels + t.offset + (size_t)y * t.w + (size_t)x);
+    return mk((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f, (float)((c >> 16) & 0xFFu) / 255.0f);
+}
+__device__ __forceinline__ const float* vtx(const DevScene& S, uint32_t vi) {
+    return reinterpret_cast<const float*>(S.verts + (size_t)vi * 44u);
+}
+// barycentric interpolation (b0 v0 + b1 v1) + b2 v2 of the uv (offset 36) of
+// three vertices
+__device__ __forceinline__ void interp_uv(const DevScene& S, const uint32_t* vi, float b0, float b1, float b2,
+                                          float& u, float& v) {
+    const float* a = vtx(S, vi[0]);
+    const float* b = vtx(S, vi[1]);
+    const float* c = vtx(S, vi[2]);
+    u = (a[9] * b0 + b[9] * b1) + c[9] * b2;
+    v = (a[10] * b0 + b[10] * b1) + c[10] * b2;
+}
+__device__ __forceinline__ bool textured(const hr_material& m) {
+    return m.textures[0] > -1.0f || m.textures[1] > -1.0f || m.textures[2] > -1.0f || m.textures[3] > -1.0f;
+}
+
+// Geometry and surface of a closest hit (hybridHit.rchit:26-68,
+// closesthit.rchit:26-70): albedo is the diffuse texel or the material colour
+// (before linearisation), emissive the emissive texel or 0, roughness and
+// metallic the g and b of the metallic-roughness texel or the material's.
 struct HitGeo {
     v3 N, V, worldPos;
     float NdotV;
     const hr_material* mat;
+    v3 albedo, emissive;
+    float roughness, metallic;
 };
 __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, const HitD& hit) {
     HitGeo g;
@@ -442,9 +474,11 @@ __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, 
     const PrimInfo& p = S.prims[S.tri_prim[hit.tri]];
     uint32_t local = hit.tri - p.tri_offset;
     uint32_t ib = p.index_base + 3 * local;
-    v3 n0 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 0]]);
-    v3 n1 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 1]]);
-    v3 n2 = ld3(S.vnormal[p.vertex_base + S.indices[ib + 2]]);
+    uint32_t vi[3] = {p.vertex_base + S.indices[ib + 0], p.vertex_base + S.indices[ib + 1],
+                      p.vertex_base + S.indices[ib + 2]};
+    v3 n0 = ld3(S.vnormal[vi[0]]);
+    v3 n1 = ld3(S.vnormal[vi[1]]);
+    v3 n2 = ld3(S.vnormal[vi[2]]);
     v3 nrm = add(add(scl(n0, b0), scl(n1, b1)), scl(n2, b2));
     const float* M = p.nm;
     g.N = normalize(mk((M[0] * nrm.x + M[1] * nrm.y) + M[2] * nrm.z, (M[3] * nrm.x + M[4] * nrm.y) + M[5] * nrm.z,
@@ -452,7 +486,23 @@ __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, 
     g.V = normalize(neg(rd));
     g.NdotV = clamp01(dot(g.N, g.V));
     g.worldPos = add(ro, scl(rd, hit.t));
-    g.mat = &S.mats[p.material];
+    const hr_material& m = S.mats[p.material];
+    g.mat = &m;
+    g.albedo = ld3p(m.diffuse_ior);
+    g.emissive = mk(0.0f, 0.0f, 0.0f);
+    g.roughness = m.shading_metal_rough_idx[2];
+    g.metallic = m.shading_metal_rough_idx[1];
+    if (textured(m)) {
+        float u, v;
+        interp_uv(S, vi, b0, b1, b2, u, v);
+        if (m.textures[0] > -1.0f) g.albedo = tex_rgba8(S, m.textures[0], u, v);
+        if (m.textures[2] > -1.0f) g.emissive = tex_rgba8(S, m.textures[2], u, v);
+        if (m.textures[3] > -1.0f) {
+            v3 rm = tex_rgba8(S, m.textures[3], u, v);
+            g.roughness = rm.y;
+            g.metallic = rm.z;
+        }
+    }
     return g;
 }
 

@@ -100,9 +100,9 @@ __device__ HitOut shade_hit(const DevScene& S, const FrameArgs& F, v3 rd, const 
     float NdotV = g.NdotV;
     v3 Lo = mk(0.0f, 0.0f, 0.0f);
     int mode = (int)mat.shading_metal_rough_idx[0];
-    v3 albedo = pow3(ld3p(mat.diffuse_ior), 2.2f);
-    float roughness = mat.shading_metal_rough_idx[2];
-    float metallic = mat.shading_metal_rough_idx[1];
+    v3 albedo = pow3(g.albedo, 2.2f);
+    float roughness = g.roughness;
+    float metallic = g.metallic;
     v3 F0 = mixs(mk(0.04f, 0.04f, 0.04f), albedo, metallic);
     v3 irradiance = env_lookup(S.irr, S.irr_w, S.irr_h, N, PI_H);
     v3 direction = mk(1.0f, 1.0f, 1.0f);
@@ -182,7 +182,7 @@ __device__ HitOut shade_hit(const DevScene& S, const FrameArgs& F, v3 rd, const 
     v3 Fa = FresnelSchlick(NdotV, F0);
     v3 kD = scl(sub(mk(1.0f, 1.0f, 1.0f), Fa), 1.0f - metallic);
     v3 ambient = mul(mul(kD, albedo), irradiance);
-    out.color = add(Lo, ambient);
+    out.color = add(add(Lo, ambient), g.emissive);
     out.dir = direction;
     out.scatter = scatter;
     return out;

@@ -80,6 +80,14 @@ def load(path=LIB_PATH):
         "hr_scene_refit": (st, [vp, vp]),
         "hr_scene_set_materials": (st, [vp, vp, u32]),
         "hr_scene_set_env": (st, [vp, C.POINTER(abi.hr_env)]),
+        "hr_scene_set_textures": (st, [vp, C.POINTER(abi.hr_texture), u32]),
+        "hr_post_gamma": (st, [vp, u32, u32, vp, vp, vp]),
+        "hr_image_load": (st, [C.c_char_p, C.POINTER(abi.hr_image), C.c_char_p, sz]),
+        "hr_image_decode": (st, [vp, sz, C.POINTER(abi.hr_image), C.c_char_p, sz]),
+        "hr_image_free": (None, [C.POINTER(abi.hr_image)]),
+        "hr_asset_load": (st, [C.c_char_p, u32, C.POINTER(vp), C.c_char_p, sz]),
+        "hr_asset_view_get": (st, [vp, C.POINTER(abi.hr_asset_view)]),
+        "hr_asset_destroy": (None, [vp]),
         "hr_scene_destroy": (None, [vp]),
         "hr_scene_info": (st, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_float)]),
         "hr_scene_export": (st, [vp, u32, vp, sz]),
@@ -161,6 +169,12 @@ class Context:
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
 
+    def post_gamma(self, width, height, src, out=None, rgba8=None):
+        """the display pass (postFragment.frag) of a float4[H][W] DeviceBuffer
+        into out (float4, may be src) and/or rgba8 (uint8[H][W][4])"""
+        self.check(self.L.hr_post_gamma(self.h, width, height, src.ptr, None if out is None else out.ptr,
+                                        None if rgba8 is None else rgba8.ptr))
+
     def set_tuning(self, blocks_per_cu=8, tile_w=8, wave_budget_mb=3072):
         t = abi.hr_tuning(blocks_per_cu, tile_w, wave_budget_mb, 0)
         self.check(self.L.hr_context_set_tuning(self.h, C.byref(t)))
@@ -225,6 +239,8 @@ class HybridScene:
         self.h = h
         ctx.own(self)
         ctx.check(ctx.L.hr_scene_set_env(h, C.byref(self.packed.env)))
+        if self.packed.texture_count:
+            ctx.check(ctx.L.hr_scene_set_textures(h, self.packed.textures, self.packed.texture_count))
 
     def close(self):
         if getattr(self, "h", None):
@@ -254,6 +270,14 @@ class HybridScene:
     def refit(self, models):
         m = np.ascontiguousarray(models, dtype=np.float32)
         self.ctx.check(self.ctx.L.hr_scene_refit(self.h, m.ctypes.data))
+
+    def set_textures(self, textures):
+        """material textures: a list of (height, width, 4) uint8 RGBA8 arrays"""
+        ts = [np.ascontiguousarray(t, dtype=np.uint8) for t in textures]
+        arr = (abi.hr_texture * max(1, len(ts)))()
+        for i, t in enumerate(ts):
+            arr[i] = abi.hr_texture(t.ctypes.data, t.shape[1], t.shape[0])
+        self.ctx.check(self.ctx.L.hr_scene_set_textures(self.h, arr, len(ts)))
 
     def set_materials(self, materials):
         m = np.ascontiguousarray(materials, dtype=np.float32)

@@ -46,6 +46,7 @@ class Scene:
     width: int
     height: int
     params: dict = field(default_factory=dict)
+    textures: list = field(default_factory=list)  # RGBA8 (height, width, 4) uint8 arrays
 
     @property
     def tri_count(self):
@@ -410,6 +411,145 @@ def make_batch(count=32, seed_base=0, tri_range=(50_000, 2_000_000), resolutions
                                       extent=extent, samples=spp, spp=spp, rough_max=1.0, adaptive=True,
                                       name="batch%02d" % k))
     return out
+
+
+# ---------------------------------------------------------------- textures
+def texture_checker(n=64, cells=8, rng=None):
+    """RGBA8 checkerboard of random cell colours, (n, n, 4) uint8."""
+    rng = rng if rng is not None else np.random.default_rng(0)
+    col = rng.integers(24, 256, size=(cells, cells, 3))
+    idx = np.arange(n) * cells // n
+    t = np.full((n, n, 4), 255, dtype=np.uint8)
+    t[..., :3] = col[idx[:, None], idx[None, :]]
+    return t
+
+
+def texture_normal_map(n=64, bumps=6, strength=3.0, rng=None):
+    """Tangent-space normal map of a sum of Gaussian bumps, encoded as the
+    shader decodes it: c = n * 127 + 128 (geometry_shader.frag:46)."""
+    rng = rng if rng is not None else np.random.default_rng(1)
+    y, x = np.mgrid[0:n, 0:n] / n
+    h = np.zeros((n, n))
+    for _ in range(bumps):
+        cx, cy, r, a = rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0.05, 0.2), rng.uniform(-1, 1)
+        h += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * r * r))
+    gy, gx = np.gradient(h, 1.0 / n)
+    nv = np.stack([-gx * strength / n * 8, -gy * strength / n * 8, np.ones_like(h)], -1)
+    nv /= np.linalg.norm(nv, axis=-1, keepdims=True)
+    t = np.full((n, n, 4), 255, dtype=np.uint8)
+    t[..., :3] = np.clip(np.rint(nv * 127 + 128), 0, 255)
+    return t
+
+
+def texture_emissive(n=32, rng=None):
+    """Emissive stripes: dim coloured bands on black."""
+    rng = rng if rng is not None else np.random.default_rng(2)
+    t = np.zeros((n, n, 4), dtype=np.uint8)
+    t[..., 3] = 255
+    band = (np.arange(n) // 4) % 2 == 0
+    t[band, :, :3] = rng.integers(40, 200, size=3)
+    return t
+
+
+def texture_metal_rough(n=32, rng=None):
+    """Metallic-roughness map as glTF lays it out: g = roughness, b = metallic."""
+    rng = rng if rng is not None else np.random.default_rng(3)
+    cells = 4
+    idx = np.arange(n) * cells // n
+    rough = rng.integers(0, 256, size=(cells, cells))
+    metal = rng.integers(0, 2, size=(cells, cells)) * 255
+    t = np.zeros((n, n, 4), dtype=np.uint8)
+    t[..., 1] = rough[idx[:, None], idx[None, :]]
+    t[..., 2] = metal[idx[:, None], idx[None, :]]
+    t[..., 3] = 255
+    return t
+
+
+def spherical_uv(V, shift=(0.0, 0.0), scale=1.0):
+    """Copy of a unit-sphere mesh's vertex array with uv = (0.5 + atan2(z, x) /
+    2pi, acos(y) / pi) * scale + shift."""
+    V = V.copy()
+    x, y, z = V[:, 0].astype(np.float64), V[:, 1].astype(np.float64), V[:, 2].astype(np.float64)
+    V[:, 9] = (0.5 + np.arctan2(z, x) / (2 * np.pi)) * scale + shift[0]
+    V[:, 10] = np.arccos(np.clip(y, -1, 1)) / np.pi * scale + shift[1]
+    return V
+
+
+def _tex_material(i, albedo, mode, metallic, roughness, tex, ior=1.0):
+    m = material(i, albedo, mode, metallic, roughness, ior)
+    m[4:8] = tex
+    return m
+
+
+def make_textured(seed=0, width=320, height=180, spheres=24, frame=0.0):
+    """Textured variant of the small scene: a ground quad with an albedo
+    checker (uv x6, so REPEAT wraps), a normal map, a metallic-roughness map
+    and vertex colours; spheres with spherical uv (some shifted to negative
+    uv) carrying albedo + normal maps, emissive stripes, glossy mirrors whose
+    roughness comes from the metallic-roughness map, glass and plain
+    materials. Textures: 0 checker 64^2, 1 normal map 64^2, 2 emissive 32^2,
+    3 metallic-roughness 32^2, 4 checker 16^2."""
+    rng = np.random.default_rng(seed)
+    ground = quad_mesh(100.0)
+    gV = ground[0].copy()
+    gV[:, 9:11] *= 6.0
+    gV[:, 6:9] = [(1.0, 0.8, 0.6), (0.6, 1.0, 0.8), (0.8, 0.6, 1.0), (1.0, 1.0, 1.0)]
+    sph = icosphere(2)
+    s1 = (spherical_uv(sph[0]), sph[1])
+    s2 = (spherical_uv(sph[0], shift=(-1.37, -2.25), scale=2.0), sph[1])
+    textures = [texture_checker(64, 8, rng), texture_normal_map(64, 6, 3.0, rng), texture_emissive(32, rng),
+                texture_metal_rough(32, rng), texture_checker(16, 4, rng)]
+    mats = [_tex_material(0, (1, 1, 1), 0, 0.0, 0.5, (0, 1, -1, 3))]
+    rows = [(0, 0, len(ground[1]), 0, np.eye(4))]
+    kinds = ["albedo_normal", "emissive", "glossy_tex", "glass", "plain", "albedo_neg"]
+    for i in range(spheres):
+        kind = kinds[i % len(kinds)]
+        alb = rng.uniform(0.2, 1.0, size=3)
+        if kind == "albedo_normal":
+            m = _tex_material(1 + i, alb, 0, 0.0, 0.5, (4, 1, -1, -1))
+        elif kind == "emissive":
+            m = _tex_material(1 + i, alb, 0, 0.0, 0.5, (-1, -1, 2, -1))
+        elif kind == "glossy_tex":
+            m = _tex_material(1 + i, alb, 3, 1.0, 0.0, (0, -1, -1, 3))
+        elif kind == "glass":
+            m = material(1 + i, alb, 4, 0.0, 0.0, ior=1.5)
+        elif kind == "plain":
+            m = material(1 + i, alb, 0, float(rng.uniform() < 0.5), float(rng.uniform(0, 1)))
+        else:
+            m = _tex_material(1 + i, alb, 0, 0.0, 0.5, (0, 1, 2, 3))
+        mats.append(m)
+        c = rng.uniform([-40, 1, -40], [40, 10, 40])
+        r = rng.uniform(1.0, 4.0)
+        rows.append((2 if kind == "albedo_neg" else 1, 0, len(sph[1]), 1 + i, translate_scale(c, r)))
+    lights = np.stack([point_light((0, 50, 10), intensity=1500.0, max_distance=500.0, radius=2.0),
+                       directional_light((0.4, 1.0, 0.3), intensity=1.0)])
+    sky, irr = make_env()
+    return Scene("textured_%dx%d" % (width, height), [(gV, ground[1]), s1, s2], _prims(rows), np.stack(mats),
+                 lights, None, rt_camera((0, 25, 70), (0, 0, 0), width, height, frame=frame), sky, irr, width,
+                 height, dict(shadow_samples=2, refl_spp=2, max_depth=3), textures=textures)
+
+
+def make_facing_quad(tex, width=64, height=64, size=8.0, uv_scale=3.0, eye_z=10.0, color=(1.0, 1.0, 1.0),
+                     albedo=(1.0, 1.0, 1.0), mode=0, metallic=0.0, roughness=0.5):
+    """One square of side `size` in the plane z = 0 facing +z, seen by a camera
+    at (0, 0, eye_z) looking at the origin, with uv rotated against the screen
+    axes: uv = ((y + size/2), (x + size/2)) / size * uv_scale. tex: the four
+    material texture slots; textures are set by the caller. Black environment,
+    no lights."""
+    h = size / 2.0
+    V = np.zeros((4, 11), dtype=np.float32)
+    V[:, 0:3] = [(-h, -h, 0), (h, -h, 0), (h, h, 0), (-h, h, 0)]
+    V[:, 3:6] = (0, 0, 1)
+    V[:, 6:9] = color
+    V[:, 9] = (V[:, 1] + h) / size * uv_scale
+    V[:, 10] = (V[:, 0] + h) / size * uv_scale
+    idx = np.array([0, 1, 2, 0, 2, 3], dtype=np.uint32)
+    mats = np.stack([_tex_material(0, albedo, mode, metallic, roughness, tex)])
+    sky = np.zeros((2, 4, 4), dtype=np.float32)
+    irr = np.zeros((2, 4, 4), dtype=np.float32)
+    return Scene("facing_quad", [(V, idx)], _prims([(0, 0, 6, 0, np.eye(4))]), mats, np.zeros((0, 12), np.float32),
+                 None, rt_camera((0, 0, eye_z), (0, 0, 0), width, height), sky, irr, width, height,
+                 dict(shadow_samples=1, refl_spp=1, max_depth=1))
 
 
 WORKLOADS = {
