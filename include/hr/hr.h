@@ -298,7 +298,8 @@ hr_status hr_scene_info(const hr_scene* scene, uint32_t* tri_count, uint32_t* no
 /* which: 0 = traversal nodes (node_count x 64 B), 1 = sorted world-space
  * triangles (tri_count x 48 B), 2 = Morton codes in sorted order (tri_count x
  * 4 B), 3 = sorted triangle ids (tri_count x 4 B), 4 = Karras internal nodes
- * (left, right, first, last) ((tri_count - 1) x 16 B). */
+ * (left, right, first, last) ((tri_count - 1) x 16 B), 5 = 4-wide traversal
+ * nodes (node_count x 128 B, slots of binary nodes at even depth). */
 hr_status hr_scene_export(const hr_scene* scene, uint32_t which, void* dst_host, size_t bytes);
 
 /* ---- per-frame passes ---------------------------------------------------- */

@@ -202,3 +202,56 @@ def test_terrain_rows_parity(gpu_ctx, oracle_lib):
     d = np.nan_to_num(np.abs(lit[sel] - olit[sel]))
     print("lit max abs err", d.max())
     assert ((d > RAD_TOL).any(-1)).mean() < GRAZE_FRAC
+
+
+def _binary_children(nodes, i):
+    n = nodes[i]
+    u = n.view(np.uint32)
+    return [(int(u[3]), n[0:3], n[4:7]), (int(u[7]), n[8:11], n[12:15])]
+
+
+@pytest.mark.parametrize("leaf_max", [1, 4])
+def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
+    """the 4-wide nodes of the timed kernels: from the root they reach exactly
+    the binary tree's leaves, each child box equal to the binary box of the
+    same subtree, empty slots with inverted boxes"""
+    sc = scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=24, soup_tris=256)
+    hr = _hr()
+    hs = hr.HybridScene(gpu_ctx, sc, leaf_max=leaf_max)
+    bn = hs.export(0).view(np.float32).reshape(-1, 16)
+    w4 = hs.export(5).reshape(-1, 32)
+    LEAF = 0x80000000
+
+    def wide(i):
+        f = w4[i].view(np.float32)
+        lo = np.stack([f[0:4], f[8:12], f[16:20]], -1)
+        hi = np.stack([f[4:8], f[12:16], f[20:24]], -1)
+        return [(int(w4[i][24 + k]), lo[k], hi[k]) for k in range(4)]
+
+    bin_leaves, wide_leaves = [], []
+    stack = [0]
+    while stack:  # binary: leaves with their boxes
+        i = stack.pop()
+        for code, lo, hi in _binary_children(bn, i):
+            if code & LEAF:
+                bin_leaves.append((code, tuple(lo), tuple(hi)))
+            else:
+                stack.append(code)
+    stack = [0]
+    internal_boxes = {}
+    for i in range(len(bn)):
+        for code, lo, hi in _binary_children(bn, i):
+            if not code & LEAF:
+                internal_boxes[code] = (tuple(lo), tuple(hi))
+    while stack:
+        i = stack.pop()
+        for code, lo, hi in wide(i):
+            if code == 0xFFFFFFFF:
+                assert np.all(lo == np.inf) and np.all(hi == -np.inf)
+            elif code & LEAF:
+                wide_leaves.append((code, tuple(lo), tuple(hi)))
+            else:
+                assert internal_boxes[code] == (tuple(lo), tuple(hi))
+                stack.append(code)
+    assert sorted(bin_leaves) == sorted(wide_leaves)
+    assert len(wide_leaves) > 100

@@ -77,6 +77,7 @@ void normal_matrix(const float* m, float* nm) {
 DevScene dev_scene(const hr_scene* s) {
     DevScene d;
     d.nodes = s->bb.nodes;
+    d.nodes4 = s->bb.nodes4;
     d.tris = s->bb.stris;
     d.tri_prim = s->bb.tri_prim;
     d.prims = s->d_prims;
@@ -571,6 +572,7 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     check(scene_alloc(s, &b.flags, nn));
     check(scene_alloc(s, &b.kbox, nn * 2));
     check(scene_alloc(s, &b.nodes, nn));
+    check(scene_alloc(s, &b.nodes4, nn));
     check(scene_alloc(s, &b.stris, n));
     check(scene_alloc(s, &b.tri_prim, n));
     check(scene_alloc(s, &b.depth, 1));
@@ -772,6 +774,7 @@ hr_status hr_scene_export(const hr_scene* s, uint32_t which, void* dst, size_t b
         case 1: src = s->bb.stris; n = (size_t)nt * sizeof(STri); break;
         case 2: src = s->bb.keys; n = (size_t)nt * 4; break;
         case 3: src = s->bb.ids; n = (size_t)nt * 4; break;
+        case 5: src = s->bb.nodes4; n = (size_t)s->node_count * sizeof(TNode4); break;
         case 4: {
             if (nt >= 2) {
                 size_t m = nt - 1;

@@ -28,6 +28,18 @@ __host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) { 
 struct __align__(64) TNode {
     float4 a, b, c, d;
 };
+// 4-wide traversal node, 128 B, one per binary node at even depth (slot =
+// the binary node's index): child k's box in component k of lo/hi x/y/z and
+// its code in code[k] (an internal slot, a leaf code, or EMPTY4 with an
+// inverted box). Built from the binary nodes by taking grandchildren; the
+// timed traversal kernels walk it, the counting kernels walk the binary one.
+constexpr uint32_t EMPTY4 = 0xFFFFFFFFu;
+constexpr uint32_t MAX_STACK4 = 3u * (MAX_DEPTH / 2u) + 1u;  // 3 pushes per 4-wide level
+struct __align__(64) TNode4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 code;
+    uint4 pad;
+};
 // Sorted world-space triangle, 48 B: (v0, triangle id), (v1, 0), (v2, 0).
 struct __align__(16) STri {
     float4 v0, v1, v2;
@@ -42,6 +54,7 @@ struct __align__(16) PrimInfo {
     uint32_t _pad[3];
 };
 static_assert(sizeof(TNode) == 64, "TNode");
+static_assert(sizeof(TNode4) == 128, "TNode4");
 static_assert(sizeof(STri) == 48, "STri");
 static_assert(sizeof(PrimInfo) == 64, "PrimInfo");
 
@@ -56,6 +69,7 @@ static_assert(sizeof(TexInfo) == 16, "TexInfo");
 // Read-only view of a built scene, passed to kernels by value.
 struct DevScene {
     const TNode* nodes;
+    const TNode4* nodes4;
     const STri* tris;
     const uint32_t* tri_prim;   // global triangle id -> primitive
     const PrimInfo* prims;
@@ -202,6 +216,7 @@ struct BuildBuffers {
     uint32_t* flags;    // refit arrival counters
     float4* kbox;       // internal node boxes x 2
     TNode* nodes;
+    TNode4* nodes4;     // node_count slots, filled at even depths
     STri* stris;
     uint32_t* tri_prim;
     uint32_t* depth;    // 1 uint

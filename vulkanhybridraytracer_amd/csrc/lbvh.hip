@@ -351,6 +351,65 @@ __global__ __launch_bounds__(256) void k_layout(BuildBuffers b) {
     b.nodes[i] = nd;
 }
 
+// 4-wide nodes: binary node i at even depth (edges from the root, counted
+// along the parent links) gets slot i holding its children, each internal
+// child replaced by that child's two children; child order is kept.
+__global__ __launch_bounds__(256) void k_collapse4(BuildBuffers b) {
+    uint32_t n = b.tri_count;
+    uint32_t nn = n >= 2 ? n - 1 : 1;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    uint32_t d = 0;
+    if (n >= 2)
+        for (uint32_t j = i; j != 0 && d <= 1024u; ++d) j = b.kparent[j];  // deeper trees fail the build's depth check
+    if (d & 1u) return;
+    const float inf = __uint_as_float(0x7f800000u);
+    float lo[3][4], hi[3][4];
+    uint32_t code[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        code[k] = EMPTY4;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a][k] = inf;
+            hi[a][k] = -inf;
+        }
+    }
+    int m = 0;
+    auto put = [&](uint32_t c, float4 l, float4 h) {
+        code[m] = c;
+        lo[0][m] = l.x; lo[1][m] = l.y; lo[2][m] = l.z;
+        hi[0][m] = h.x; hi[1][m] = h.y; hi[2][m] = h.z;
+        ++m;
+    };
+    const TNode nd = b.nodes[i];
+    uint32_t c0 = __float_as_uint(nd.a.w), c1 = __float_as_uint(nd.b.w);
+    if (c0 & LEAF_BIT) {
+        put(c0, nd.a, nd.b);
+    } else {
+        const TNode g = b.nodes[c0];
+        put(__float_as_uint(g.a.w), g.a, g.b);
+        put(__float_as_uint(g.b.w), g.c, g.d);
+    }
+    if (c1 & LEAF_BIT) {
+        put(c1, nd.c, nd.d);
+    } else {
+        const TNode g = b.nodes[c1];
+        put(__float_as_uint(g.a.w), g.a, g.b);
+        put(__float_as_uint(g.b.w), g.c, g.d);
+    }
+    TNode4 q;
+    q.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    q.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    q.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    q.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    q.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    q.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    q.code = make_uint4(code[0], code[1], code[2], code[3]);
+    q.pad = make_uint4(0u, 0u, 0u, 0u);
+    b.nodes4[i] = q;
+}
+
 __global__ __launch_bounds__(256) void k_gather(BuildBuffers b) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= b.tri_count) return;
@@ -412,6 +471,7 @@ hipError_t finish(BuildBuffers& b, hipStream_t st) {
     }
     uint32_t nn = n >= 2 ? n - 1 : 1;
     k_layout<<<blocks(nn, 256), 256, 0, st>>>(b);
+    k_collapse4<<<blocks(nn, 256), 256, 0, st>>>(b);
     k_gather<<<blocks(n, 256), 256, 0, st>>>(b);
     k_fill_u32<<<1, 64, 0, st>>>(b.depth, 1, 0u);
     k_depth<<<blocks(n, 256), 256, 0, st>>>(b);

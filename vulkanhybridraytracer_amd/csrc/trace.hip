@@ -195,7 +195,11 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
     float tmax = 0.0f;
     const float tmin = 0.001f;
     int sp = 0;
-    uint32_t spill[STACK_SPILL];
+    // The timed kernel walks the 4-wide nodes (up to 3 pushes per level), the
+    // counting kernel the binary ones.
+    constexpr uint32_t STACK_CAP = COUNT ? MAX_DEPTH : MAX_STACK4;
+    uint32_t spill[STACK_CAP - STACK_LDS];
+    const TNode4* nodes4 = jobs[0].S.nodes4;
     // Speculative traversal (Aila & Laine 2009) in the timed kernel: the
     // shadow result does not depend on the order leaves are tested in. The
     // counting kernel keeps the canonical near-first order, so its node and
@@ -302,6 +306,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
                     target = F.adaptive ? (hard_light(F, li) ? 1u : 4u) : SS;
                     if (BATCH) {
                         nodes = J.S.nodes;
+                        nodes4 = J.S.nodes4;
                         tris = J.S.tris;
                         node_count = J.S.node_count;
                     }
@@ -328,7 +333,59 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
             cur = node_count ? 0u : DONE;
             continue;
         }
-        while (cur < LEAF_BIT) {
+        while (!COUNT && cur < LEAF_BIT) {
+            // 4-wide node: box tests, hits sorted near to far, the nearest
+            // taken next and the others pushed far first
+            const float4* np = reinterpret_cast<const float4*>(nodes4 + cur);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 cd = reinterpret_cast<const uint4*>(np)[6];
+            const float inf = __uint_as_float(0x7f800000u);
+            float t0, t1, t2, t3;
+            const bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, t0) && cd.x != EMPTY4;
+            const bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, tmin, tmax, t1) && cd.y != EMPTY4;
+            const bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, tmin, tmax, t2) && cd.z != EMPTY4;
+            const bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, tmin, tmax, t3) && cd.w != EMPTY4;
+            float k0 = b0 ? t0 : inf, k1 = b1 ? t1 : inf, k2 = b2 ? t2 : inf, k3 = b3 ? t3 : inf;
+            uint32_t c0 = cd.x, c1 = cd.y, c2 = cd.z, c3 = cd.w;
+            sort_pair(k0, c0, k1, c1);
+            sort_pair(k2, c2, k3, c3);
+            sort_pair(k0, c0, k2, c2);
+            sort_pair(k1, c1, k3, c3);
+            sort_pair(k1, c1, k2, c2);
+            const uint32_t nh = (uint32_t)b0 + (uint32_t)b1 + (uint32_t)b2 + (uint32_t)b3;
+            auto push = [&](uint32_t code) {
+                if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
+                else if (sp < (int)STACK_CAP) spill[sp - STACK_LDS] = code;
+                else c.overflow = 1;
+                sp = min(sp + 1, (int)STACK_CAP);
+            };
+            if (nh == 0u) {
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            } else {
+                if (nh > 3u) push(c3);
+                if (nh > 2u) push(c2);
+                if (nh > 1u) push(c1);
+                cur = c0;
+            }
+            // postpone the first leaf found and keep walking until every lane
+            // still in this loop holds one
+            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                leaf = cur;
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+            if (__ballot(leaf == 0u) == 0ull) break;
+        }
+        while (COUNT && cur < LEAF_BIT) {
             const float4* np = reinterpret_cast<const float4*>(nodes + cur);
             float4 a = np[0], b = np[1], cc = np[2], d = np[3];
             if (COUNT) ++c.nodes;
@@ -343,9 +400,9 @@ __global__ __launch_bounds__(BLOCK) void k_shadows(const ShadowJob* __restrict__
                     farc = c0;
                 }
                 if (sp < (int)STACK_LDS) lds[sp * BLOCK] = farc;
-                else if (sp < (int)MAX_DEPTH) spill[sp - STACK_LDS] = farc;
+                else if (sp < (int)STACK_CAP) spill[sp - STACK_LDS] = farc;
                 else c.overflow = 1;
-                sp = min(sp + 1, (int)MAX_DEPTH);
+                sp = min(sp + 1, (int)STACK_CAP);
                 cur = nearc;
             } else if (h0) {
                 cur = c0;

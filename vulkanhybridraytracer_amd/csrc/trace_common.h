@@ -38,6 +38,17 @@ struct HitD {
     uint32_t tri;
 };
 
+// compare-exchange of (key, code) pairs, ascending by key (selects, no branch)
+__device__ __forceinline__ void sort_pair(float& ka, uint32_t& ca, float& kb, uint32_t& cb) {
+    const bool sw = kb < ka;
+    const float k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    const uint32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
+    ka = k0;
+    kb = k1;
+    ca = c0;
+    cb = c1;
+}
+
 template <bool ANY, bool COUNT>
 __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tmax, uint32_t* lds, HitD& h,
                          Cnt& c) {

@@ -262,7 +262,8 @@ class HybridScene:
         info = self.info()
         n = info["tri_count"]
         size, dt = {0: (info["node_count"], abi.NODE_DTYPE), 1: (n, abi.STRI_DTYPE), 2: (n, np.uint32),
-                    3: (n, np.uint32), 4: (max(n - 1, 0) * 4, np.uint32)}[which]
+                    3: (n, np.uint32), 4: (max(n - 1, 0) * 4, np.uint32),
+                    5: (info["node_count"] * 32, np.uint32)}[which]
         out = np.zeros(size, dtype=dt)
         self.ctx.check(self.ctx.L.hr_scene_export(self.h, which, out.ctypes.data, out.nbytes))
         return out
