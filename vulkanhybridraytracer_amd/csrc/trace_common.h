@@ -49,9 +49,106 @@ __device__ __forceinline__ void sort_pair(float& ka, uint32_t& ca, float& kb, ui
     cb = c1;
 }
 
+constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (fewer than 2^27 triangles)
+
+// One step over a 4-wide node: its children's boxes are tested against
+// [tmin, tmax], the hits sorted near to far; all but the nearest are pushed
+// (far first, so the nearer ones pop first) and the nearest is returned. With
+// no hit the next entry is popped (DONE when the stack is empty). The stack
+// holds STACK_LDS entries per lane in LDS and the rest in `spill`, CAP in
+// all; a push beyond CAP sets the overflow flag.
+template <uint32_t CAP>
+__device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur, const RayP& r, float tmin,
+                                              float tmax, uint32_t* lds, uint32_t* spill, int& sp, Cnt& c) {
+    const float4* np = reinterpret_cast<const float4*>(nodes4 + cur);
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const uint4 cd = reinterpret_cast<const uint4*>(np)[6];
+    const float inf = __uint_as_float(0x7f800000u);
+    float t0, t1, t2, t3;
+    const bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, t0) && cd.x != EMPTY4;
+    const bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, r, tmin, tmax, t1) && cd.y != EMPTY4;
+    const bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, r, tmin, tmax, t2) && cd.z != EMPTY4;
+    const bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, r, tmin, tmax, t3) && cd.w != EMPTY4;
+    float k0 = b0 ? t0 : inf, k1 = b1 ? t1 : inf, k2 = b2 ? t2 : inf, k3 = b3 ? t3 : inf;
+    uint32_t c0 = cd.x, c1 = cd.y, c2 = cd.z, c3 = cd.w;
+    sort_pair(k0, c0, k1, c1);
+    sort_pair(k2, c2, k3, c3);
+    sort_pair(k0, c0, k2, c2);
+    sort_pair(k1, c1, k3, c3);
+    sort_pair(k1, c1, k2, c2);
+    const uint32_t nh = (uint32_t)b0 + (uint32_t)b1 + (uint32_t)b2 + (uint32_t)b3;
+    if (nh == 0u) {
+        if (sp == 0) return DONE;
+        --sp;
+        return sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+    }
+    auto push = [&](uint32_t code) {
+        if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
+        else if (sp < (int)CAP) spill[sp - STACK_LDS] = code;
+        else c.overflow = 1;
+        sp = min(sp + 1, (int)CAP);
+    };
+    if (nh > 3u) push(c3);
+    if (nh > 2u) push(c2);
+    if (nh > 1u) push(c1);
+    return c0;
+}
+
+// Closest-hit (ANY = false) or any-hit traversal of one ray. The counting
+// variant walks the binary nodes in near-first order (the oracle's order and
+// counts); the timed variant walks the 4-wide nodes. Results are the same:
+// any hit is any hit, and the closest hit is the smallest (t, triangle id).
 template <bool ANY, bool COUNT>
 __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tmax, uint32_t* lds, HitD& h,
                          Cnt& c) {
+    if (!COUNT) {
+        if (S.node_count == 0) return false;
+        uint32_t spill[MAX_STACK4 - STACK_LDS];
+        int sp = 0;
+        uint32_t cur = 0;
+        float tbest = tmax;
+        uint32_t best = 0xFFFFFFFFu;
+        float bu = 0.0f, bv = 0.0f;
+        for (;;) {
+            if (cur < LEAF_BIT) {
+                cur = wide_step<MAX_STACK4>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
+                continue;
+            }
+            if (cur == DONE) break;
+            uint32_t first = cur & 0x07FFFFFFu;
+            uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tbest, t, b1, b2)) {
+                    uint32_t id = __float_as_uint(v0.w);
+                    if (ANY) {
+                        h.t = t;
+                        h.u = b1;
+                        h.v = b2;
+                        h.tri = id;
+                        return true;
+                    }
+                    if (t < tbest || id < best) {
+                        tbest = t;
+                        best = id;
+                        bu = b1;
+                        bv = b2;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            --sp;
+            cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+        }
+        if (best == 0xFFFFFFFFu) return false;
+        h.t = tbest;
+        h.u = bu;
+        h.v = bv;
+        h.tri = best;
+        return true;
+    }
     if (S.node_count == 0) return false;
     uint32_t spill[STACK_SPILL];
     int sp = 0;
@@ -303,14 +400,15 @@ __device__ __forceinline__ uint32_t wave_take(uint32_t* next) {
 template <bool ANY, bool COUNT, class GetRay, class PutHit>
 __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint32_t* lds, Cnt& c, GetRay get,
                              PutHit put) {
-    constexpr uint32_t DONE = 0xFFFFFFFFu;  // no leaf code reaches it (fewer than 2^27 triangles)
     uint32_t cur = DONE, idx = 0;
     bool have = false;
     RayP r;
     float tmin = 0.0f, tbest = 0.0f, bu = 0.0f, bv = 0.0f;
     uint32_t best = 0xFFFFFFFFu;
     int sp = 0;
-    uint32_t spill[STACK_SPILL];
+    // the timed variant walks the 4-wide nodes, the counting one the binary nodes
+    constexpr uint32_t STACK_CAP = COUNT ? MAX_DEPTH : MAX_STACK4;
+    uint32_t spill[STACK_CAP - STACK_LDS];
     // speculative traversal (postponed leaves) except when counting: the
     // closest hit (smallest t, then smallest triangle id) and any-hit results
     // do not depend on the order leaves are tested in
@@ -347,7 +445,22 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
             cur = S.node_count ? 0u : DONE;
             continue;
         }
-        while (cur < LEAF_BIT) {
+        while (!COUNT && cur < LEAF_BIT) {
+            cur = wide_step<STACK_CAP>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
+            // postpone the first leaf found and keep walking until every lane
+            // still in this loop holds one
+            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                leaf = cur;
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+            if (__ballot(leaf == 0u) == 0ull) break;
+        }
+        while (COUNT && cur < LEAF_BIT) {
             const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
             float4 a = np[0], b = np[1], cc = np[2], d = np[3];
             if (COUNT) ++c.nodes;
@@ -362,9 +475,9 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
                     farc = c0;
                 }
                 if (sp < (int)STACK_LDS) lds[sp * BLOCK] = farc;
-                else if (sp < (int)MAX_DEPTH) spill[sp - STACK_LDS] = farc;
+                else if (sp < (int)STACK_CAP) spill[sp - STACK_LDS] = farc;
                 else c.overflow = 1;
-                sp = min(sp + 1, (int)MAX_DEPTH);
+                sp = min(sp + 1, (int)STACK_CAP);
                 cur = nearc;
             } else if (h0) {
                 cur = c0;
