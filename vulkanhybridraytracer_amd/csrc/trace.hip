@@ -407,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             uint32_t first = lf & 0x07FFFFFFu;
             uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
             for (uint32_t k = first; k < first + cnt; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(tris + k);
+                const gf4 tp = gptr(tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 if (COUNT) ++c.tris;
                 float t, b1, b2;

@@ -51,6 +51,19 @@ __device__ __forceinline__ void sort_pair(float& ka, uint32_t& ca, float& kb, ui
 
 constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (fewer than 2^27 triangles)
 
+// Node and triangle arrays live in global memory; reading them through
+// global-address-space pointers gives global (not flat) loads, whose waits do
+// not also wait for the LDS stack traffic.
+// (The host compilation pass of this file sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const float4 __attribute__((address_space(1)))* gf4;
+typedef const uint4 __attribute__((address_space(1)))* gu4;
+#else
+typedef const float4* gf4;
+typedef const uint4* gu4;
+#endif
+__device__ __forceinline__ gf4 gptr(const void* p) { return (gf4)(p); }
+
 // One step over a 4-wide node: its children's boxes are tested against
 // [tmin, tmax], the hits sorted near to far; all but the nearest are pushed
 // (far first, so the nearer ones pop first) and the nearest is returned. With
@@ -60,9 +73,9 @@ constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (few
 template <uint32_t CAP>
 __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur, const RayP& r, float tmin,
                                               float tmax, uint32_t* lds, uint32_t* spill, int& sp, Cnt& c) {
-    const float4* np = reinterpret_cast<const float4*>(nodes4 + cur);
+    const gf4 np = gptr(nodes4 + cur);
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const uint4 cd = reinterpret_cast<const uint4*>(np)[6];
+    const uint4 cd = ((gu4)np)[6];
     const float inf = __uint_as_float(0x7f800000u);
     float t0, t1, t2, t3;
     const bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, t0) && cd.x != EMPTY4;
@@ -118,7 +131,7 @@ __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tma
             uint32_t first = cur & 0x07FFFFFFu;
             uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
             for (uint32_t k = first; k < first + cnt; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                const gf4 tp = gptr(S.tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, b1, b2;
                 if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tbest, t, b1, b2)) {
@@ -195,7 +208,7 @@ __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tma
             uint32_t first = cur & 0x07FFFFFFu;
             uint32_t cnt = ((cur >> 27) & 0xFu) + 1u;
             for (uint32_t k = first; k < first + cnt; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                const gf4 tp = gptr(S.tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 if (COUNT) ++c.tris;
                 float t, b1, b2;
@@ -516,7 +529,7 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
             uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
             bool stop = false;
             for (uint32_t k = first; k < first + cnt; ++k) {
-                const float4* tp = reinterpret_cast<const float4*>(S.tris + k);
+                const gf4 tp = gptr(S.tris + k);
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 if (COUNT) ++c.tris;
                 float t, b1, b2;
