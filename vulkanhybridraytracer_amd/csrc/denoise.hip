@@ -48,6 +48,52 @@ __host__ __device__ __forceinline__ size_t frag_index(uint32_t co, uint32_t k, u
 constexpr int CONV_MT = 4;  // 16-pixel tiles per wave
 constexpr int CONV_NT = 4;  // 16-channel tiles per wave
 
+// Epilogue of both convolution kernels: bias, ReLU, and 4 channels per lane
+// stored into the output's channel slice (BF16 8-byte or FP32 16-byte writes).
+__device__ __forceinline__ void conv_store(const ConvArgs& a, const f32x4 (&acc)[CONV_MT][CONV_NT],
+                                           const uint32_t (&pix)[CONV_MT], const bool (&pok)[CONV_MT],
+                                           uint32_t s0, uint32_t nsub, uint32_t lane) {
+    // C layout: row (channel) = (lane / 16) * 4 + r, column (pixel) = lane % 16
+    const bool vec_ok = ((a.out_C | a.out_off) & 3u) == 0;  // 4-channel groups aligned
+#pragma unroll
+    for (int s = 0; s < CONV_NT; ++s) {
+        if ((uint32_t)s >= nsub) continue;
+        const uint32_t co = (s0 + s) * 16u + (lane >> 4) * 4u;
+        if (co >= a.Cout) continue;
+        float b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) b[r] = (a.bias && co + r < a.Cout) ? a.bias[co + r] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < CONV_MT; ++m) {
+            if (!pok[m]) continue;
+            const uint32_t p = pix[m];
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = acc[m][s][r] + b[r];
+                if (a.relu) v[r] = fmaxf(v[r], 0.0f);
+            }
+            const size_t o = (size_t)p * a.out_C + a.out_off + co;
+            if (a.out_f32) {
+                if (vec_ok && co + 4u <= a.Cout) {
+                    *reinterpret_cast<float4*>(a.out_f32 + o) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+                    for (int r = 0; r < 4; ++r)
+                        if (co + r < a.Cout) a.out_f32[o + r] = v[r];
+                }
+            } else {
+                if (vec_ok && co + 4u <= a.Cout) {
+                    bf16x4 q = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
+                    *reinterpret_cast<bf16x4*>(a.out + o) = q;
+                } else {
+                    for (int r = 0; r < 4; ++r)
+                        if (co + r < a.Cout) a.out[o + r] = f2bf(v[r]);
+                }
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -110,45 +156,111 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
             ++tap;
         }
     }
-    // C layout: row (channel) = (lane / 16) * 4 + r, column (pixel) = lane % 16
-    const bool vec_ok = ((a.out_C | a.out_off) & 3u) == 0;  // 4-channel groups aligned
+    uint32_t pix[CONV_MT];
+    bool pok[CONV_MT];
 #pragma unroll
-    for (int s = 0; s < CONV_NT; ++s) {
-        if ((uint32_t)s >= nsub) continue;
-        const uint32_t co = (s0 + s) * 16u + (lane >> 4) * 4u;
-        if (co >= a.Cout) continue;
-        float b[4];
+    for (int m = 0; m < CONV_MT; ++m) {
+        pix[m] = pix0 + m * 16u + (lane & 15u);
+        pok[m] = pix[m] < P;
+    }
+    conv_store(a, acc, pix, pok, s0, nsub, lane);
+}
+
+// The same convolution with the input staged in LDS: a workgroup computes an
+// 8 x 32 pixel tile (wave w: rows 2w, 2w + 1) and first copies the 10 x 34
+// input halo, all Cin channels, into LDS (zero outside the image; pixel
+// stride Cin + 8 elements against bank conflicts), so the 9 taps read their
+// fragments from LDS instead of re-reading global memory. Same K walk,
+// fragments and MFMA order per K step as k_conv.
+constexpr int LT_H = 8, LT_W = 32, HALO_H = LT_H + 2, HALO_W = LT_W + 2;
+
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+    extern __shared__ __align__(16) __bf16 tile[];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t ps = a.Cin + 8u;
+    uint32_t b = blockIdx.x;
+    const uint32_t tx = b % tiles_x;
+    b /= tiles_x;
+    const uint32_t ty = b % tiles_y;
+    const uint32_t n = b / tiles_y;
+    const int x0 = (int)(tx * LT_W), y0 = (int)(ty * LT_H);
+    const int H = (int)a.H, W = (int)a.W;
+    const uint32_t cpp = a.Cin >> 3;  // 16-byte chunks per pixel
+    const uint32_t total = (uint32_t)(HALO_H * HALO_W) * cpp;
+    const bf16x8 zero = {};
+    for (uint32_t i = threadIdx.x; i < total; i += 256u) {
+        const uint32_t hp = i / cpp, ch = i - hp * cpp;
+        const int y = y0 + (int)(hp / HALO_W) - 1, x = x0 + (int)(hp % HALO_W) - 1;
+        bf16x8 v = zero;
+        if (y >= 0 && y < H && x >= 0 && x < W)
+            v = *reinterpret_cast<const bf16x8*>(a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C +
+                                                 a.in_off + ch * 8u);
+        *reinterpret_cast<bf16x8*>(tile + (size_t)hp * ps + ch * 8u) = v;
+    }
+    __syncthreads();
+    const uint32_t s0 = blockIdx.y * CONV_NT;
+    const uint32_t ns_total = a.Cout_pad >> 4;
+    const uint32_t nsub = min((uint32_t)CONV_NT, ns_total - s0);
+    f32x4 acc[CONV_MT][CONV_NT];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) b[r] = (a.bias && co + r < a.Cout) ? a.bias[co + r] : 0.0f;
+    for (int m = 0; m < CONV_MT; ++m)
+#pragma unroll
+        for (int s = 0; s < CONV_NT; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int rbase[CONV_MT];  // LDS pixel index of this lane's pixel for tap (0, 0) minus one row and column
+    uint32_t pix[CONV_MT];
+    bool pok[CONV_MT];
+#pragma unroll
+    for (int m = 0; m < CONV_MT; ++m) {
+        const int ry = (int)(2u * w) + (m >> 1), rx = (m & 1) * 16 + (int)(lane & 15u);
+        rbase[m] = (ry + 1) * HALO_W + rx + 1;
+        const int y = y0 + ry, x = x0 + rx;
+        pok[m] = y < H && x < W;
+        pix[m] = pok[m] ? (n * a.H + (uint32_t)y) * a.W + (uint32_t)x : 0u;
+    }
+    uint32_t ci = (lane >> 4) * 8u, tap = 0;
+    while (ci >= a.Cin) {
+        ci -= a.Cin;
+        ++tap;
+    }
+    const bf16x8* wfrag = reinterpret_cast<const bf16x8*>(a.wt) + (size_t)s0 * 64u + lane;
+    const size_t wstep = (size_t)ns_total * 64u;
+    // weight fragments one K step ahead, so their global latency overlaps
+    // the current step's MFMAs
+    bf16x8 wcur[CONV_NT];
+#pragma unroll
+    for (int s = 0; s < CONV_NT; ++s) wcur[s] = (uint32_t)s < nsub ? wfrag[s * 64] : zero;
+    for (uint32_t kk = 0; kk < a.Kpad; kk += 32u) {
+        const bool kval = tap < 9u;
+        const uint32_t tc = kval ? tap : 4u;  // padding K: any in-tile address, value unused
+        const int doff = ((int)(tc / 3u) - 1) * HALO_W + (int)(tc % 3u) - 1;
+        bf16x8 av[CONV_MT];
 #pragma unroll
         for (int m = 0; m < CONV_MT; ++m) {
-            const uint32_t p = pix0 + m * 16u + (lane & 15u);
-            if (p >= P) continue;
-            float v[4];
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (size_t)(rbase[m] + doff) * ps + ci);
+            av[m] = kval ? v : zero;
+        }
+        bf16x8 wnext[CONV_NT];
+        const bool more = kk + 32u < a.Kpad;
+        const bf16x8* wk = wfrag + (size_t)((kk >> 5) + 1u) * wstep;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                v[r] = acc[m][s][r] + b[r];
-                if (a.relu) v[r] = fmaxf(v[r], 0.0f);
-            }
-            const size_t o = (size_t)p * a.out_C + a.out_off + co;
-            if (a.out_f32) {
-                if (vec_ok && co + 4u <= a.Cout) {
-                    *reinterpret_cast<float4*>(a.out_f32 + o) = make_float4(v[0], v[1], v[2], v[3]);
-                } else {
-                    for (int r = 0; r < 4; ++r)
-                        if (co + r < a.Cout) a.out_f32[o + r] = v[r];
-                }
-            } else {
-                if (vec_ok && co + 4u <= a.Cout) {
-                    bf16x4 q = {f2bf(v[0]), f2bf(v[1]), f2bf(v[2]), f2bf(v[3])};
-                    *reinterpret_cast<bf16x4*>(a.out + o) = q;
-                } else {
-                    for (int r = 0; r < 4; ++r)
-                        if (co + r < a.Cout) a.out[o + r] = f2bf(v[r]);
-                }
+        for (int s = 0; s < CONV_NT; ++s) wnext[s] = (more && (uint32_t)s < nsub) ? wk[s * 64] : zero;
+#pragma unroll
+        for (int s = 0; s < CONV_NT; ++s) {
+            if ((uint32_t)s < nsub) {
+#pragma unroll
+                for (int m = 0; m < CONV_MT; ++m)
+                    acc[m][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wcur[s], av[m], acc[m][s], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int s = 0; s < CONV_NT; ++s) wcur[s] = wnext[s];
+        ci += 32u;
+        while (ci >= a.Cin && tap < 9u) {
+            ci -= a.Cin;
+            ++tap;
+        }
     }
+    conv_store(a, acc, pix, pok, s0, nsub, lane);
 }
 
 // ---- weight gradient --------------------------------------------------------------
@@ -299,6 +411,78 @@ __global__ void k_up_bwd(const __bf16* gout, uint32_t g_C, uint32_t g_off, __bf1
     const __bf16* b = gout + (((size_t)n * Ho + 2 * y) * Wo + 2 * x) * g_C + g_off + c;
     float s = (bf2f(b[0]) + bf2f(b[g_C])) + (bf2f(b[(size_t)Wo * g_C]) + bf2f(b[(size_t)Wo * g_C + g_C]));
     gin[i] = f2bf(s);
+}
+
+// The same four operations on 8 channels per thread (16-byte loads and
+// stores; per element the arithmetic above), for channel counts and offsets
+// that are multiples of 8.
+__global__ void k_pool8(const __bf16* in, uint32_t in_C, uint32_t in_off, __bf16* out, uint32_t C, uint32_t N,
+                        uint32_t Ho, uint32_t Wo) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t C8 = C >> 3;
+    size_t total = (size_t)N * Ho * Wo * C8;
+    if (i >= total) return;
+    uint32_t c = (uint32_t)(i % C8) * 8u;
+    size_t q = i / C8;
+    uint32_t x = q % Wo, y = (q / Wo) % Ho, n = q / ((size_t)Wo * Ho);
+    uint32_t Wi = Wo * 2, Hi = Ho * 2;
+    const __bf16* b = in + (((size_t)n * Hi + 2 * y) * Wi + 2 * x) * in_C + in_off + c;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(b), a1 = *reinterpret_cast<const bf16x8*>(b + in_C);
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(b + (size_t)Wi * in_C);
+    const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(b + (size_t)Wi * in_C + in_C);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = f2bf(0.25f * ((bf2f(a0[k]) + bf2f(a1[k])) + (bf2f(a2[k]) + bf2f(a3[k]))));
+    *reinterpret_cast<bf16x8*>(out + q * C + c) = o;
+}
+__global__ void k_pool_bwd_add8(const __bf16* gout, __bf16* gin, uint32_t gin_C, uint32_t gin_off, uint32_t C,
+                                uint32_t N, uint32_t Ho, uint32_t Wo) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t Hi = Ho * 2, Wi = Wo * 2;
+    const uint32_t C8 = C >> 3;
+    size_t total = (size_t)N * Hi * Wi * C8;
+    if (i >= total) return;
+    uint32_t c = (uint32_t)(i % C8) * 8u;
+    size_t q = i / C8;
+    uint32_t x = q % Wi, y = (q / Wi) % Hi, n = q / ((size_t)Wi * Hi);
+    const bf16x8 g = *reinterpret_cast<const bf16x8*>(gout + (((size_t)n * Ho + y / 2) * Wo + x / 2) * C + c);
+    bf16x8* d = reinterpret_cast<bf16x8*>(gin + q * gin_C + gin_off + c);
+    bf16x8 v = *d;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = f2bf(bf2f(v[k]) + 0.25f * bf2f(g[k]));
+    *d = v;
+}
+__global__ void k_up8(const __bf16* in, __bf16* out, uint32_t out_C, uint32_t out_off, uint32_t C, uint32_t N,
+                      uint32_t Ho, uint32_t Wo) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t C8 = C >> 3;
+    size_t total = (size_t)N * Ho * Wo * C8;
+    if (i >= total) return;
+    uint32_t c = (uint32_t)(i % C8) * 8u;
+    size_t q = i / C8;
+    uint32_t x = q % Wo, y = (q / Wo) % Ho, n = q / ((size_t)Wo * Ho);
+    uint32_t Hi = Ho / 2, Wi = Wo / 2;
+    *reinterpret_cast<bf16x8*>(out + q * out_C + out_off + c) =
+        *reinterpret_cast<const bf16x8*>(in + (((size_t)n * Hi + y / 2) * Wi + x / 2) * C + c);
+}
+__global__ void k_up_bwd8(const __bf16* gout, uint32_t g_C, uint32_t g_off, __bf16* gin, uint32_t C, uint32_t N,
+                          uint32_t Hi, uint32_t Wi) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t C8 = C >> 3;
+    size_t total = (size_t)N * Hi * Wi * C8;
+    if (i >= total) return;
+    uint32_t c = (uint32_t)(i % C8) * 8u;
+    size_t q = i / C8;
+    uint32_t x = q % Wi, y = (q / Wi) % Hi, n = q / ((size_t)Wi * Hi);
+    uint32_t Wo = Wi * 2, Ho = Hi * 2;
+    const __bf16* b = gout + (((size_t)n * Ho + 2 * y) * Wo + 2 * x) * g_C + g_off + c;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(b), a1 = *reinterpret_cast<const bf16x8*>(b + g_C);
+    const bf16x8 a2 = *reinterpret_cast<const bf16x8*>(b + (size_t)Wo * g_C);
+    const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(b + (size_t)Wo * g_C + g_C);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = f2bf((bf2f(a0[k]) + bf2f(a1[k])) + (bf2f(a2[k]) + bf2f(a3[k])));
+    *reinterpret_cast<bf16x8*>(gin + q * C + c) = o;
 }
 
 // input features of image n (16 channels, 9 used; zero outside the frame):
@@ -464,8 +648,17 @@ __global__ void k_pack_weights(PackArgs a) {
 static inline uint32_t nblk(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
+    const uint32_t ny = ((a.Cout_pad >> 4) + dn::CONV_NT - 1) / dn::CONV_NT;
+    // LDS-staged halo when the tile fits in 64 KiB and 16-byte chunks stay aligned
+    const size_t lds = (size_t)dn::HALO_H * dn::HALO_W * (a.Cin + 8u) * sizeof(__bf16);
+    if (lds <= 65536u && (a.Cin & 7u) == 0 && (a.in_C & 7u) == 0 && (a.in_off & 7u) == 0) {
+        const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
+        dim3 grid(a.N * tx * ty, ny);
+        dn::k_conv_lds<<<grid, 256, lds, st>>>(a, tx, ty);
+        return hipGetLastError();
+    }
     size_t P = (size_t)a.N * a.H * a.W;
-    dim3 grid(nblk(P, 4 * 16 * dn::CONV_MT), ((a.Cout_pad >> 4) + dn::CONV_NT - 1) / dn::CONV_NT);
+    dim3 grid(nblk(P, 4 * 16 * dn::CONV_MT), ny);
     dn::k_conv<<<grid, 256, 0, st>>>(a);
     return hipGetLastError();
 }
@@ -492,22 +685,36 @@ hipError_t dn_relu_bwd(__bf16* g, uint32_t g_C, uint32_t g_off, const __bf16* ac
 }
 hipError_t dn_pool(const __bf16* in, uint32_t in_C, uint32_t in_off, __bf16* out, uint32_t C, uint32_t N, uint32_t Ho,
                    uint32_t Wo, hipStream_t st) {
-    dn::k_pool<<<nblk((size_t)N * Ho * Wo * C, 256), 256, 0, st>>>(in, in_C, in_off, out, C, N, Ho, Wo);
+    if (((C | in_C | in_off) & 7u) == 0)
+        dn::k_pool8<<<nblk((size_t)N * Ho * Wo * (C >> 3), 256), 256, 0, st>>>(in, in_C, in_off, out, C, N, Ho, Wo);
+    else
+        dn::k_pool<<<nblk((size_t)N * Ho * Wo * C, 256), 256, 0, st>>>(in, in_C, in_off, out, C, N, Ho, Wo);
     return hipGetLastError();
 }
 hipError_t dn_pool_bwd_add(const __bf16* gout, __bf16* gin, uint32_t gin_C, uint32_t gin_off, uint32_t C, uint32_t N,
                            uint32_t Ho, uint32_t Wo, hipStream_t st) {
-    dn::k_pool_bwd_add<<<nblk((size_t)N * Ho * Wo * 4 * C, 256), 256, 0, st>>>(gout, gin, gin_C, gin_off, C, N, Ho, Wo);
+    if (((C | gin_C | gin_off) & 7u) == 0)
+        dn::k_pool_bwd_add8<<<nblk((size_t)N * Ho * Wo * 4 * (C >> 3), 256), 256, 0, st>>>(gout, gin, gin_C, gin_off, C,
+                                                                                          N, Ho, Wo);
+    else
+        dn::k_pool_bwd_add<<<nblk((size_t)N * Ho * Wo * 4 * C, 256), 256, 0, st>>>(gout, gin, gin_C, gin_off, C, N, Ho,
+                                                                                  Wo);
     return hipGetLastError();
 }
 hipError_t dn_up(const __bf16* in, __bf16* out, uint32_t out_C, uint32_t out_off, uint32_t C, uint32_t N, uint32_t Ho,
                  uint32_t Wo, hipStream_t st) {
-    dn::k_up<<<nblk((size_t)N * Ho * Wo * C, 256), 256, 0, st>>>(in, out, out_C, out_off, C, N, Ho, Wo);
+    if (((C | out_C | out_off) & 7u) == 0)
+        dn::k_up8<<<nblk((size_t)N * Ho * Wo * (C >> 3), 256), 256, 0, st>>>(in, out, out_C, out_off, C, N, Ho, Wo);
+    else
+        dn::k_up<<<nblk((size_t)N * Ho * Wo * C, 256), 256, 0, st>>>(in, out, out_C, out_off, C, N, Ho, Wo);
     return hipGetLastError();
 }
 hipError_t dn_up_bwd(const __bf16* gout, uint32_t g_C, uint32_t g_off, __bf16* gin, uint32_t C, uint32_t N, uint32_t Hi,
                      uint32_t Wi, hipStream_t st) {
-    dn::k_up_bwd<<<nblk((size_t)N * Hi * Wi * C, 256), 256, 0, st>>>(gout, g_C, g_off, gin, C, N, Hi, Wi);
+    if (((C | g_C | g_off) & 7u) == 0)
+        dn::k_up_bwd8<<<nblk((size_t)N * Hi * Wi * (C >> 3), 256), 256, 0, st>>>(gout, g_C, g_off, gin, C, N, Hi, Wi);
+    else
+        dn::k_up_bwd<<<nblk((size_t)N * Hi * Wi * C, 256), 256, 0, st>>>(gout, g_C, g_off, gin, C, N, Hi, Wi);
     return hipGetLastError();
 }
 hipError_t dn_features(const FeatArgs& a, uint32_t n, hipStream_t st) {
