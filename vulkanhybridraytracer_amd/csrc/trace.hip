@@ -466,12 +466,200 @@ __global__ void k_reset_u32(uint32_t* p) {
     if (threadIdx.x < SHADOW_QUEUES) p[threadIdx.x] = 0u;
 }
 
+__global__ void k_fill_bits(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
 inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u) * ((f.height + 15u) / 16u); }
+
+
+// Sample-parallel shadow pass for one frame without adaptive sampling and a
+// power-of-two sample count S in [4, 64]: the work items are (group, sample)
+// pairs, numbered so a group's S samples are consecutive, and a lane traces
+// one item at a time. Lanes that take work together take consecutive items,
+// i.e. the nearly parallel rays of one pixel towards one light, which keeps a
+// wave's rays coherent. An unoccluded sample sets bit `item` of a bit array
+// (atomicOr: the result does not depend on the order); k_shadow_bits turns
+// each group's bits into the factor count / S. Items are fetched from the
+// same per-XCD bands as in k_shadows.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadows_spl(
+    const ShadowJob* __restrict__ jobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits,
+    unsigned long long* counters) {
+    __shared__ uint32_t stk[STACK_LDS * BLOCK];
+    uint32_t* lds = stk + threadIdx.x;
+    Cnt c = {};
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
+    uint32_t emptied = 0;
+    const ShadowJob& J = jobs[0];
+    const FrameArgs& F = J.F;
+    const TNode4* nodes4 = J.S.nodes4;
+    const STri* tris = J.S.tris;
+    const uint32_t node_count = J.S.node_count;
+    const uint32_t SS = 1u << lg2s;
+    uint32_t item = 0;
+    bool tracing = false, hit = false;
+    uint32_t cur = DONE, leaf = 0u;
+    RayP r;
+    float tmax = 0.0f;
+    const float tmin = 0.001f;
+    int sp = 0;
+    uint32_t spill[MAX_STACK4 - STACK_LDS];
+    for (;;) {
+        if (cur == DONE && leaf == 0u) {
+            if (tracing) {  // the ray of `item` ended
+                if (hit) ++c.occluded;
+                else atomicOr(bits + (item >> 5), 1u << (item & 31u));
+                tracing = false;
+            }
+            bool exhausted = false;
+            for (;;) {
+                uint32_t k = 0;
+                for (uint32_t b = 1; b <= SHADOW_QUEUES; ++b)
+                    if (__ballot(emptied >= b)) k = b;
+                emptied = k;
+                if (k >= SHADOW_QUEUES) {
+                    exhausted = true;
+                    break;
+                }
+                const uint32_t band = (home + k) & (SHADOW_QUEUES - 1u);
+                const uint32_t band_end = (uint32_t)(((uint64_t)nitems * (band + 1u)) / SHADOW_QUEUES);
+                unsigned long long need = __ballot(1);
+                uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+                uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(next_item + band, (uint32_t)__popcll(need));
+                base = (uint32_t)__shfl((int)base, (int)leader, 64);
+                item = (uint32_t)(((uint64_t)nitems * band) / SHADOW_QUEUES) + base + rank;
+                if (item >= band_end) {
+                    emptied = k + 1u;
+                    continue;
+                }
+                const uint32_t g = item >> lg2s, s = item & (SS - 1u);
+                const uint32_t per_tile = F.nl * 64u;
+                const uint32_t t = g / per_tile, rem = g - t * per_tile;
+                const uint32_t li = rem >> 6, q = rem & 63u;
+                const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+                const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+                if (x >= F.width || y >= F.height) continue;
+                const size_t p = (size_t)y * F.width + x;
+                const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
+                const bool background = P4.w == 0.0f && N4.w == 0.0f;
+                if (background) continue;  // factor 0: no bit
+                const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
+                if (mode == 4) {  // glass: factor 1
+                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
+                    continue;
+                }
+                const v3 P = ld3(P4);
+                const v3 N = normalize(ld3(N4));
+                const hr_light& lt = F.lights[li];
+                const bool isDir = lt.pos_maxdist[3] < 0.0f;
+                v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
+                const float dist = length(L);
+                L = normalize(L);
+                const float NdotL = clamp01(dot(N, L));
+                if (!(NdotL > 0.0f)) continue;  // factor 0
+                if (!(isDir || dist < lt.pos_maxdist[3])) {  // beyond the light's reach: factor 1
+                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
+                    continue;
+                }
+                const uint32_t seed = tea(y * F.width + x, F.frame);
+                Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
+                const v3 dir = light_sample_dir(F, li, P, L, gs, s, SS);
+                tmax = isDir ? 1000.0f : dist + 1.0f;
+                r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
+                ++c.shadow;
+                tracing = true;
+                hit = false;
+                sp = 0;
+                cur = node_count ? 0u : DONE;
+                break;
+            }
+            if (exhausted) break;
+            continue;
+        }
+        while (cur < LEAF_BIT) {
+            cur = wide_step<MAX_STACK4>(nodes4, cur, r, tmin, tmax, lds, spill, sp, c);
+            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                leaf = cur;
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+            if (__ballot(leaf == 0u) == 0ull) break;
+        }
+        uint32_t lf = DONE;
+        bool in_order = false;
+        if (leaf != 0u) {
+            lf = leaf;
+            leaf = 0u;
+        } else if (cur >= LEAF_BIT && cur != DONE) {
+            lf = cur;
+            in_order = true;
+        }
+        if (lf != DONE) {
+            const uint32_t first = lf & 0x07FFFFFFu;
+            const uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const gf4 tp = gptr(tris + k);
+                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2)) {
+                    hit = true;
+                    break;
+                }
+            }
+            if (hit) {
+                cur = DONE;
+                leaf = 0u;
+            } else if (in_order) {
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+        }
+    }
+    flush_counters(c, counters, false);
+}
+
+// shadow factor of every group from the bits of its samples
+__global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict__ jobs, uint32_t ngroups,
+                                                     uint32_t lg2s, const uint32_t* __restrict__ bits) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const ShadowJob& J = jobs[0];
+    const FrameArgs& F = J.F;
+    const uint32_t per_tile = F.nl * 64u;
+    const uint32_t t = g / per_tile, rem = g - t * per_tile;
+    const uint32_t li = rem >> 6, q = rem & 63u;
+    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    if (x >= F.width || y >= F.height) return;
+    const uint32_t SS = 1u << lg2s;
+    const uint64_t b0 = (uint64_t)g << lg2s;
+    uint32_t n;
+    if (SS == 64u) {
+        n = (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
+    } else {
+        const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
+        n = (uint32_t)__popc(w & ((1u << SS) - 1u));
+    }
+    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)SS;
+}
 
 }  // namespace
 
@@ -490,6 +678,21 @@ uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w) {
     uint32_t tiles = ((f.width + tw - 1u) / tw) * ((f.height + th - 1u) / th);
     return tiles * f.nl * 64u;
 }
+
+hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, unsigned long long* counters,
+                              hipStream_t st, uint32_t* next_item, uint32_t* bits, uint32_t resident_blocks) {
+    const uint64_t items = (uint64_t)ngroups << lg2s;
+    if (items >= (1ull << 32)) return hipErrorInvalidValue;
+    const uint32_t words = (uint32_t)((items + 31u) >> 5);
+    uint32_t nb = min(resident_blocks, (uint32_t)((items + BLOCK - 1) / BLOCK));
+    if (nb == 0) return hipSuccess;
+    k_reset_u32<<<1, 64, 0, st>>>(next_item);
+    k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
+    k_shadows_spl<<<nb, BLOCK, 0, st>>>(jobs_dev, (uint32_t)items, lg2s, next_item, bits, counters);
+    k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, lg2s, bits);
+    return hipGetLastError();
+}
+
 hipError_t launch_shadows(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, unsigned long long* counters,
                           bool count, hipStream_t st, uint32_t* next_group, uint32_t resident_blocks) {
     uint32_t nb = min(resident_blocks, (ngroups + BLOCK - 1) / BLOCK);
