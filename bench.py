@@ -295,7 +295,9 @@ def main():
     traffic, traffic_src = load_traffic(sc.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "k_shadows", "algorithmic_bytes": int(bytes_shadow),
+                "kernel": "k_shadows_spl" if (not sc.params.get("flags", 0) & 4 and
+                                              sc.params["shadow_samples"] in (4, 8, 16, 32, 64)) else "k_shadows",
+                "algorithmic_bytes": int(bytes_shadow),
                 "avg_launch_ms": round(avg_shadow_ms, 3),
                 "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),
                 "tris_per_ray": round(shadow_tris / max(shadow_rays, 1), 2)}
