@@ -847,17 +847,17 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     std::vector<ShadowJob> jobs;
     uint32_t ngroups = make_jobs(ctx, plans, jobs);
     HR_TRY(ensure_jobs(ctx, 1));
-    // sample-parallel kernel: timed, non-adaptive frames with 4..64 samples (a power of two)
+    // sample-parallel kernel: timed frames with 4..64 samples (a power of two)
     uint32_t S = f.samples, lg2s = 0;
     while ((1u << lg2s) < S) ++lg2s;
-    const bool spl = !count && !f.adaptive && S >= 4u && S <= 64u && (1u << lg2s) == S &&
-                     ((uint64_t)ngroups << lg2s) < (1ull << 32);
-    if (spl) HR_TRY(ensure_work(ctx, ((((size_t)ngroups << lg2s) + 31u) / 32u) * 4u));
+    const bool spl = !count && S >= 4u && S <= 64u && (1u << lg2s) == S && ((uint64_t)ngroups << lg2s) < (1ull << 32);
+    if (spl) HR_TRY(ensure_work(ctx, shadows_spl_scratch(ngroups, lg2s, f.adaptive != 0u)));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     if (spl)
-        HR_HIP(ctx, launch_shadows_spl(ctx->d_jobs, ngroups, lg2s, ctx->d_counters, ctx->stream, ctx->d_next_group,
-                                       reinterpret_cast<uint32_t*>(ctx->work), ctx->resident_blocks));
+        HR_HIP(ctx, launch_shadows_spl(ctx->d_jobs, ngroups, lg2s, f.adaptive != 0u, ctx->d_counters, ctx->stream,
+                                       ctx->d_next_group, reinterpret_cast<uint32_t*>(ctx->work),
+                                       ctx->resident_blocks));
     else
         HR_HIP(ctx, launch_shadows(ctx->d_jobs, 1, ngroups, ctx->d_counters, count, ctx->stream, ctx->d_next_group,
                                    ctx->resident_blocks));

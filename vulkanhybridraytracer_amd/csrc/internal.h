@@ -139,10 +139,13 @@ struct ShadowJob {
     uint32_t _pad;
 };
 uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w);
-// One frame, no adaptive sampling, S = 2^lg2s samples (4..64): one lane per
-// (group, sample) item; bits: (groups * S + 31) / 32 device words of scratch.
-hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, unsigned long long* counters,
-                              hipStream_t st, uint32_t* next_item, uint32_t* bits, uint32_t resident_blocks);
+// One frame, S = 2^lg2s samples (4..64; with adaptive sampling the penumbra
+// sample count): one lane per (group, sample) item. scratch: device memory of
+// shadows_spl_scratch() bytes.
+size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive);
+hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, bool adaptive,
+                              unsigned long long* counters, hipStream_t st, uint32_t* next_item, uint32_t* scratch,
+                              uint32_t resident_blocks);
 // jobs_dev: njobs ShadowJob in device memory; next_group: SHADOW_QUEUES device
 // uint32 band counters (reset by the launch); resident_blocks: the persistent
 // grid size

@@ -488,9 +488,16 @@ inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u
 // (atomicOr: the result does not depend on the order); k_shadow_bits turns
 // each group's bits into the factor count / S. Items are fetched from the
 // same per-XCD bands as in k_shadows.
+// MODE 0: S samples per group (no adaptive sampling). With adaptive sampling
+// the same kernel runs twice: MODE 1 traces the probes (4 per group, 1 for a
+// hard light; items numbered with 4 per group), k_shadow_decide settles the
+// groups whose probes agree and lists the others, and MODE 2 traces S refine
+// samples for each listed group (item = list slot * S + sample; the item count
+// is read from *list_count on the device).
+template <int MODE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits,
-    unsigned long long* counters) {
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
@@ -504,6 +511,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const STri* tris = J.S.tris;
     const uint32_t node_count = J.S.node_count;
     const uint32_t SS = 1u << lg2s;
+    if (MODE == 2) nitems = *list_count << lg2s;
+    const uint32_t ilg = MODE == 1 ? 2u : lg2s;  // items per group (log2)
     uint32_t item = 0;
     bool tracing = false, hit = false;
     uint32_t cur = DONE, leaf = 0u;
@@ -542,13 +551,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     emptied = k + 1u;
                     continue;
                 }
-                const uint32_t g = item >> lg2s, s = item & (SS - 1u);
+                const uint32_t g = MODE == 2 ? list[item >> ilg] : item >> ilg;
+                const uint32_t s = item & ((1u << ilg) - 1u);
                 const uint32_t per_tile = F.nl * 64u;
                 const uint32_t t = g / per_tile, rem = g - t * per_tile;
                 const uint32_t li = rem >> 6, q = rem & 63u;
                 const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
                 const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
                 if (x >= F.width || y >= F.height) continue;
+                const bool hard = MODE == 1 && hard_light(F, li);
+                if (hard && s > 0u) continue;  // a hard light takes one probe
                 const size_t p = (size_t)y * F.width + x;
                 const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
                 const bool background = P4.w == 0.0f && N4.w == 0.0f;
@@ -572,8 +584,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     continue;
                 }
                 const uint32_t seed = tea(y * F.width + x, F.frame);
-                Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
-                const v3 dir = light_sample_dir(F, li, P, L, gs, s, SS);
+                v3 dir;
+                if (MODE == 0) {
+                    Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
+                    dir = light_sample_dir(F, li, P, L, gs, s, SS);
+                } else {  // adaptive: probes at index li * 128 + s, refine samples at li * 128 + 64 + s
+                    Pcg gs = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (MODE == 2 ? 64u + s : s));
+                    dir = light_sample_dir_strat(F, li, P, L, gs, s, MODE == 2 ? SS : (hard ? 1u : 4u));
+                }
                 tmax = isDir ? 1000.0f : dist + 1.0f;
                 r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
                 ++c.shadow;
@@ -661,6 +679,63 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
     J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)SS;
 }
 
+// Adaptive sampling, after the probes: a hard light's factor is its one
+// probe; 4 agreeing probes (or S <= 4) give count / 4; the other groups are
+// appended to the refine list.
+__global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restrict__ jobs, uint32_t ngroups,
+                                                       uint32_t SS, const uint32_t* __restrict__ bits,
+                                                       uint32_t* list, uint32_t* list_count) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const ShadowJob& J = jobs[0];
+    const FrameArgs& F = J.F;
+    const uint32_t per_tile = F.nl * 64u;
+    const uint32_t t = g / per_tile, rem = g - t * per_tile;
+    const uint32_t li = rem >> 6, q = rem & 63u;
+    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    if (x >= F.width || y >= F.height) return;
+    const uint32_t w = (bits[g >> 3] >> ((g & 7u) * 4u)) & 15u;  // the 4 probe bits of group g
+    float* out = J.shadow + li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
+    if (hard_light(F, li)) {
+        *out = (float)(w & 1u) / 1.0f;
+        return;
+    }
+    const uint32_t n = (uint32_t)__popc(w);
+    if (n != 0u && n != 4u && SS > 4u) {
+        list[atomicAdd(list_count, 1u)] = g;
+        return;
+    }
+    *out = (float)n / 4.0f;
+}
+
+// factors of the refined groups
+__global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __restrict__ jobs, uint32_t lg2s,
+                                                            const uint32_t* __restrict__ bits,
+                                                            const uint32_t* __restrict__ list,
+                                                            const uint32_t* __restrict__ list_count) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= *list_count) return;
+    const ShadowJob& J = jobs[0];
+    const FrameArgs& F = J.F;
+    const uint32_t g = list[slot];
+    const uint32_t per_tile = F.nl * 64u;
+    const uint32_t t = g / per_tile, rem = g - t * per_tile;
+    const uint32_t li = rem >> 6, q = rem & 63u;
+    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    const uint32_t SS = 1u << lg2s;
+    const uint64_t b0 = (uint64_t)slot << lg2s;
+    uint32_t n;
+    if (SS == 64u) {
+        n = (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
+    } else {
+        const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
+        n = (uint32_t)__popc(w & ((1u << SS) - 1u));
+    }
+    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)SS;
+}
+
 }  // namespace
 
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st) {
@@ -679,17 +754,45 @@ uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w) {
     return tiles * f.nl * 64u;
 }
 
-hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, unsigned long long* counters,
-                              hipStream_t st, uint32_t* next_item, uint32_t* bits, uint32_t resident_blocks) {
+size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
+    const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
+    if (!adaptive) return w;
+    return w + ((((size_t)ngroups << 2) + 31u) >> 5) * 4u + (size_t)ngroups * 4u + 64u;
+}
+
+hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, bool adaptive,
+                              unsigned long long* counters, hipStream_t st, uint32_t* next_item, uint32_t* scratch,
+                              uint32_t resident_blocks) {
     const uint64_t items = (uint64_t)ngroups << lg2s;
     if (items >= (1ull << 32)) return hipErrorInvalidValue;
     const uint32_t words = (uint32_t)((items + 31u) >> 5);
     uint32_t nb = min(resident_blocks, (uint32_t)((items + BLOCK - 1) / BLOCK));
     if (nb == 0) return hipSuccess;
+    uint32_t* bits = scratch;
+    if (!adaptive) {
+        k_reset_u32<<<1, 64, 0, st>>>(next_item);
+        k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
+        k_shadows_spl<0><<<nb, BLOCK, 0, st>>>(jobs_dev, (uint32_t)items, lg2s, next_item, bits, nullptr, nullptr,
+                                                counters);
+        k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, lg2s, bits);
+        return hipGetLastError();
+    }
+    // scratch: refine bits (S per group at most), probe bits (4 per group), list, count
+    uint32_t* pbits = bits + words;
+    const uint32_t pwords = (uint32_t)((((uint64_t)ngroups << 2) + 31u) >> 5);
+    uint32_t* list = pbits + pwords;
+    uint32_t* count = list + ngroups;
+    const uint32_t pitems = ngroups << 2;
+    const uint32_t nbp = min(resident_blocks, (pitems + BLOCK - 1) / BLOCK);
+    k_reset_u32<<<1, 64, 0, st>>>(next_item);
+    k_fill_bits<<<(pwords + 255u) / 256u, 256, 0, st>>>(pbits, pwords);
+    k_fill_bits<<<1, 64, 0, st>>>(count, 1u);
+    k_shadows_spl<1><<<nbp, BLOCK, 0, st>>>(jobs_dev, pitems, lg2s, next_item, pbits, nullptr, nullptr, counters);
+    k_shadow_decide<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, 1u << lg2s, pbits, list, count);
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-    k_shadows_spl<<<nb, BLOCK, 0, st>>>(jobs_dev, (uint32_t)items, lg2s, next_item, bits, counters);
-    k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, lg2s, bits);
+    k_shadows_spl<2><<<nb, BLOCK, 0, st>>>(jobs_dev, 0u, lg2s, next_item, bits, list, count, counters);
+    k_shadow_refine_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, lg2s, bits, list, count);
     return hipGetLastError();
 }
 
