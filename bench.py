@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--cpu-rows-step", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--leaf-max", type=int, default=4, help="largest subtree stored as one BVH leaf")
+    ap.add_argument("--wave-budget-mb", type=int, default=0,
+                    help="reflection workspace budget in MiB (0: the library default)")
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--autotune", action="store_true", help="tune launch shapes before timing (batch workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -249,6 +251,9 @@ def main():
     log("scene %s: %d triangles, %dx%d, generated in %.1fs" % (sc.name, sc.tri_count, sc.width, sc.height,
                                                               time.time() - t0))
     ctx = hr.Context(0)
+    if args.wave_budget_mb:
+        t = ctx.get_tuning()
+        ctx.set_tuning(t["blocks_per_cu"], t["tile_w"], args.wave_budget_mb)
     hs = hr.HybridScene(ctx, sc, leaf_max=args.leaf_max)
     info = hs.info()
     log("LBVH build %.2f ms, depth %d" % (info["ms_build"], info["depth"]))

@@ -13,7 +13,7 @@ import os
 DEFAULT_CANDIDATES = {
     "blocks_per_cu": [4, 6, 8, 12],
     "tile_w": [4, 8, 16],
-    "wave_budget_mb": [512, 3072],
+    "wave_budget_mb": [3072, 16384, 40960],
 }
 
 

@@ -343,7 +343,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
 // ---- bounce stages -------------------------------------------------------------
 // closest hit of every queued path (persistent, dynamic fetch)
 template <bool COUNT>
-__global__ __launch_bounds__(WF_BLOCK) void k_wf_trace(DevScene S, const float4* paths, const uint32_t* count,
+__global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_wf_trace(DevScene S, const float4* paths, const uint32_t* count,
                                                        float4* hits, uint32_t* next, unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * WF_BLOCK];
     uint32_t* lds = stk + threadIdx.x;
