@@ -177,6 +177,31 @@ def test_adaptive_parity(gpu_ctx, oracle_lib):
     assert st["shadow_rays"] == oc_full["shadow_rays"]
 
 
+@pytest.mark.parametrize("samples,adaptive", [(4, False), (8, False), (32, False), (64, False),
+                                              (8, True), (32, True)])
+def test_shadow_sample_counts_parity(gpu_ctx, oracle_lib, samples, adaptive):
+    """Every sample count the sample-parallel shadow kernels take (powers of
+    two 4..64, fixed and adaptive; the group windows are parts of a word, one
+    word or two words) against the oracle, with the exact shadow-ray count."""
+    hr = _hr()
+    sc = scenes.make_terrain_scene(seed=2, width=160, height=96, quads=96, instances=24, samples=samples,
+                                   spp=1, adaptive=adaptive)
+    hs, os_ = _pair(gpu_ctx, oracle_lib, sc)
+    rd = hr.HybridRenderer(gpu_ctx, hs)
+    rd.gbuffer_primary()
+    st = abi.hr_stats()
+    rd.trace_shadows(st)
+    gb = rd.gb.download()
+    sh = rd.download_shadow()
+    osh, oc = os_.trace_shadows(gb)
+    frac, _ = _mismatch_frac(sh, osh)
+    print("S=%d adaptive=%s shadow mismatch %g soft %g" % (samples, adaptive, frac,
+                                                           float(((sh > 0) & (sh < 1)).mean())))
+    assert frac < GRAZE_FRAC
+    assert ((sh > 0) & (sh < 1)).any()
+    assert st.shadow_rays == oc["shadow_rays"]
+
+
 def test_terrain_rows_parity(gpu_ctx, oracle_lib):
     """1080p-generator scene at 480x270 with all features (rect lights, 16
     stratified samples, GGX 4 spp): every 8th row against the oracle."""

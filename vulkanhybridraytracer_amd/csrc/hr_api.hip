@@ -352,6 +352,47 @@ uint32_t make_jobs(const hr_context* ctx, const std::vector<FramePlan>& plans, s
     return base;
 }
 
+// The sample-parallel shadow kernels take timed frames whose sample counts are
+// powers of two in 4..64, all adaptive or none; items are spaced by the
+// largest count. Other batches use the group-per-lane kernel.
+struct SplPlan {
+    bool use = false;
+    bool adaptive = false;
+    uint32_t lg2s = 0;
+    size_t scratch = 0;
+};
+
+SplPlan plan_spl(const std::vector<FramePlan>& plans, uint32_t ngroups) {
+    SplPlan sp;
+    uint32_t smax = 0;
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const FrameArgs& f = plans[i].f;
+        const uint32_t S = f.samples;
+        if (plans[i].count || S < 4u || S > 64u || (S & (S - 1u)) != 0u) return sp;
+        if (i > 0 && (f.adaptive != 0u) != sp.adaptive) return sp;
+        sp.adaptive = f.adaptive != 0u;
+        smax = S > smax ? S : smax;
+    }
+    while ((1u << sp.lg2s) < smax) ++sp.lg2s;
+    if (((uint64_t)ngroups << sp.lg2s) >= (1ull << 32)) return sp;
+    sp.scratch = shadows_spl_scratch(ngroups, sp.lg2s, sp.adaptive);
+    sp.use = true;
+    return sp;
+}
+
+// Kernels of a shadow pass (device jobs already uploaded); scratch holds
+// sp.scratch bytes when sp.use.
+hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, uint32_t ngroups, const SplPlan& sp,
+                          bool count, uint32_t* next_group, uint32_t* scratch) {
+    if (sp.use)
+        HR_HIP(ctx, launch_shadows_spl(d_jobs, n, ngroups, sp.lg2s, sp.adaptive, ctx->d_counters, ctx->stream,
+                                       next_group, scratch, ctx->resident_blocks));
+    else
+        HR_HIP(ctx, launch_shadows(d_jobs, n, ngroups, ctx->d_counters, count, ctx->stream, next_group,
+                                   ctx->resident_blocks));
+    return HR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -847,20 +888,12 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     std::vector<ShadowJob> jobs;
     uint32_t ngroups = make_jobs(ctx, plans, jobs);
     HR_TRY(ensure_jobs(ctx, 1));
-    // sample-parallel kernel: timed frames with 4..64 samples (a power of two)
-    uint32_t S = f.samples, lg2s = 0;
-    while ((1u << lg2s) < S) ++lg2s;
-    const bool spl = !count && S >= 4u && S <= 64u && (1u << lg2s) == S && ((uint64_t)ngroups << lg2s) < (1ull << 32);
-    if (spl) HR_TRY(ensure_work(ctx, shadows_spl_scratch(ngroups, lg2s, f.adaptive != 0u)));
+    const SplPlan sp = plan_spl(plans, ngroups);
+    if (sp.use) HR_TRY(ensure_work(ctx, sp.scratch));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
-    if (spl)
-        HR_HIP(ctx, launch_shadows_spl(ctx->d_jobs, ngroups, lg2s, f.adaptive != 0u, ctx->d_counters, ctx->stream,
-                                       ctx->d_next_group, reinterpret_cast<uint32_t*>(ctx->work),
-                                       ctx->resident_blocks));
-    else
-        HR_HIP(ctx, launch_shadows(ctx->d_jobs, 1, ngroups, ctx->d_counters, count, ctx->stream, ctx->d_next_group,
-                                   ctx->resident_blocks));
+    HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, 1, ngroups, sp, count, ctx->d_next_group,
+                           reinterpret_cast<uint32_t*>(ctx->work)));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -926,14 +959,6 @@ hr_status plan_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, b
     return HR_OK;
 }
 
-// Kernels of a batch's shadow pass (device jobs already uploaded).
-hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, uint32_t ngroups, bool count,
-                          uint32_t* next_group) {
-    HR_HIP(ctx, launch_shadows(d_jobs, n, ngroups, ctx->d_counters, count, ctx->stream, next_group,
-                               ctx->resident_blocks));
-    return HR_OK;
-}
-
 // Kernels of a batch's shading pass, frames one after another on a shared
 // workspace at base (laid out per frame).
 hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base) {
@@ -979,9 +1004,12 @@ hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, u
     bool count = false;
     for (auto& fp : plans) count = count || fp.count;
     HR_TRY(ensure_jobs(ctx, n));
+    const SplPlan sp = plan_spl(plans, ngroups);
+    if (sp.use) HR_TRY(ensure_work(ctx, sp.scratch));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
-    HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, n, ngroups, count, ctx->d_next_group));
+    HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, n, ngroups, sp, count, ctx->d_next_group,
+                           reinterpret_cast<uint32_t*>(ctx->work)));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1021,6 +1049,10 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
     bool count = false;
     for (auto& fp : plans) count = count || fp.count;
+    // the shadow pass's scratch and the shading workspace share g->work: the
+    // passes run one after the other on the stream
+    const SplPlan sp = plan_spl(plans, ngroups);
+    if (sp.use && sp.scratch > bytes) bytes = sp.scratch;
     HR_HIP(ctx, hipSetDevice(ctx->device));
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hr_graph* g = new hr_graph();
@@ -1042,7 +1074,8 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     hr_status st = HR_OK;
     hipError_t le = launch_zero_u64(ctx->d_counters, C_COUNT, ctx->stream);
     if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
-    if (st == HR_OK) st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, count, g->d_next);
+    if (st == HR_OK)
+        st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work));
     if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work));
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);

@@ -139,13 +139,14 @@ struct ShadowJob {
     uint32_t _pad;
 };
 uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w);
-// One frame, S = 2^lg2s samples (4..64; with adaptive sampling the penumbra
-// sample count): one lane per (group, sample) item. scratch: device memory of
+// Frames with power-of-two sample counts in 4..64 (with adaptive sampling the
+// penumbra counts), all adaptive or none: one lane per (group, sample) item,
+// 2^lg2s = the largest count. scratch: device memory of
 // shadows_spl_scratch() bytes.
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive);
-hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, bool adaptive,
-                              unsigned long long* counters, hipStream_t st, uint32_t* next_item, uint32_t* scratch,
-                              uint32_t resident_blocks);
+hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
+                              bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
+                              uint32_t* scratch, uint32_t resident_blocks);
 // jobs_dev: njobs ShadowJob in device memory; next_group: SHADOW_QUEUES device
 // uint32 band counters (reset by the launch); resident_blocks: the persistent
 // grid size

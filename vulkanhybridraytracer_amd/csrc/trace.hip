@@ -494,10 +494,23 @@ inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u
 // groups whose probes agree and lists the others, and MODE 2 traces S refine
 // samples for each listed group (item = list slot * S + sample; the item count
 // is read from *list_count on the device).
-template <int MODE>
+// Several frames (njobs > 1): groups are numbered frame after frame (the
+// frames' group_base), every frame uses the stride 2^lg2s (the largest sample
+// count) and skips samples at or beyond its own count.
+__device__ __forceinline__ uint32_t job_of(const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t g) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].group_base <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int MODE, bool MULTI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadows_spl(
-    const ShadowJob* __restrict__ jobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits,
-    const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, unsigned long long* counters) {
+    const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
+    uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+    unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
@@ -505,12 +518,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
     uint32_t emptied = 0;
-    const ShadowJob& J = jobs[0];
-    const FrameArgs& F = J.F;
-    const TNode4* nodes4 = J.S.nodes4;
-    const STri* tris = J.S.tris;
-    const uint32_t node_count = J.S.node_count;
-    const uint32_t SS = 1u << lg2s;
+    const TNode4* nodes4 = jobs[0].S.nodes4;
+    const STri* tris = jobs[0].S.tris;
+    uint32_t node_count = jobs[0].S.node_count;
     if (MODE == 2) nitems = *list_count << lg2s;
     const uint32_t ilg = MODE == 1 ? 2u : lg2s;  // items per group (log2)
     uint32_t item = 0;
@@ -551,8 +561,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     emptied = k + 1u;
                     continue;
                 }
-                const uint32_t g = MODE == 2 ? list[item >> ilg] : item >> ilg;
+                const uint32_t G = MODE == 2 ? list[item >> ilg] : item >> ilg;  // global group
                 const uint32_t s = item & ((1u << ilg) - 1u);
+                const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
+                const FrameArgs& F = J.F;
+                const uint32_t SS = F.samples;
+                if (MULTI && MODE != 1 && s >= SS) continue;  // beyond this frame's sample count
+                const uint32_t g = G - J.group_base;
                 const uint32_t per_tile = F.nl * 64u;
                 const uint32_t t = g / per_tile, rem = g - t * per_tile;
                 const uint32_t li = rem >> 6, q = rem & 63u;
@@ -595,6 +610,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                 tmax = isDir ? 1000.0f : dist + 1.0f;
                 r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
                 ++c.shadow;
+                if (MULTI) {  // one frame: the scene stays uniform across the wave
+                    nodes4 = J.S.nodes4;
+                    tris = J.S.tris;
+                    node_count = J.S.node_count;
+                }
                 tracing = true;
                 hit = false;
                 sp = 0;
@@ -654,48 +674,54 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     flush_counters(c, counters, false);
 }
 
+// the number of set bits of a group's item window (stride 2^lg2s, 2 <= lg2s <= 6):
+// whole words for 32 and 64, a masked part of one word below (a shift by 32
+// would not give a full mask)
+__device__ __forceinline__ uint32_t window_bits(const uint32_t* __restrict__ bits, uint64_t b0, uint32_t lg2s) {
+    if (lg2s == 6u) return (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
+    if (lg2s == 5u) return (uint32_t)__popc(bits[b0 >> 5]);
+    const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
+    return (uint32_t)__popc(w & ((1u << (1u << lg2s)) - 1u));
+}
+
 // shadow factor of every group from the bits of its samples
-__global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict__ jobs, uint32_t ngroups,
-                                                     uint32_t lg2s, const uint32_t* __restrict__ bits) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const ShadowJob& J = jobs[0];
+__global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                     uint32_t ngroups, uint32_t lg2s,
+                                                     const uint32_t* __restrict__ bits) {
+    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= ngroups) return;
+    const ShadowJob& J = jobs[njobs > 1u ? job_of(jobs, njobs, G) : 0u];
     const FrameArgs& F = J.F;
+    const uint32_t g = G - J.group_base;
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
     const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
     const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
     if (x >= F.width || y >= F.height) return;
-    const uint32_t SS = 1u << lg2s;
-    const uint64_t b0 = (uint64_t)g << lg2s;
-    uint32_t n;
-    if (SS == 64u) {
-        n = (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
-    } else {
-        const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
-        n = (uint32_t)__popc(w & ((1u << SS) - 1u));
-    }
-    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)SS;
+    const uint32_t n = window_bits(bits, (uint64_t)G << lg2s, lg2s);
+    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
 }
 
 // Adaptive sampling, after the probes: a hard light's factor is its one
 // probe; 4 agreeing probes (or S <= 4) give count / 4; the other groups are
 // appended to the refine list.
-__global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restrict__ jobs, uint32_t ngroups,
-                                                       uint32_t SS, const uint32_t* __restrict__ bits,
+__global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                       uint32_t ngroups, const uint32_t* __restrict__ bits,
                                                        uint32_t* list, uint32_t* list_count) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const ShadowJob& J = jobs[0];
+    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= ngroups) return;
+    const ShadowJob& J = jobs[njobs > 1u ? job_of(jobs, njobs, G) : 0u];
     const FrameArgs& F = J.F;
+    const uint32_t SS = F.samples;
+    const uint32_t g = G - J.group_base;
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
     const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
     const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
     if (x >= F.width || y >= F.height) return;
-    const uint32_t w = (bits[g >> 3] >> ((g & 7u) * 4u)) & 15u;  // the 4 probe bits of group g
+    const uint32_t w = (bits[G >> 3] >> ((G & 7u) * 4u)) & 15u;  // the 4 probe bits of the group
     float* out = J.shadow + li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
     if (hard_light(F, li)) {
         *out = (float)(w & 1u) / 1.0f;
@@ -703,37 +729,30 @@ __global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restri
     }
     const uint32_t n = (uint32_t)__popc(w);
     if (n != 0u && n != 4u && SS > 4u) {
-        list[atomicAdd(list_count, 1u)] = g;
+        list[atomicAdd(list_count, 1u)] = G;
         return;
     }
     *out = (float)n / 4.0f;
 }
 
 // factors of the refined groups
-__global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __restrict__ jobs, uint32_t lg2s,
-                                                            const uint32_t* __restrict__ bits,
+__global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                            uint32_t lg2s, const uint32_t* __restrict__ bits,
                                                             const uint32_t* __restrict__ list,
                                                             const uint32_t* __restrict__ list_count) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= *list_count) return;
-    const ShadowJob& J = jobs[0];
+    const uint32_t G = list[slot];
+    const ShadowJob& J = jobs[njobs > 1u ? job_of(jobs, njobs, G) : 0u];
     const FrameArgs& F = J.F;
-    const uint32_t g = list[slot];
+    const uint32_t g = G - J.group_base;
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
     const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
     const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
-    const uint32_t SS = 1u << lg2s;
-    const uint64_t b0 = (uint64_t)slot << lg2s;
-    uint32_t n;
-    if (SS == 64u) {
-        n = (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
-    } else {
-        const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
-        n = (uint32_t)__popc(w & ((1u << SS) - 1u));
-    }
-    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)SS;
+    const uint32_t n = window_bits(bits, (uint64_t)slot << lg2s, lg2s);
+    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
 }
 
 }  // namespace
@@ -760,9 +779,9 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     return w + ((((size_t)ngroups << 2) + 31u) >> 5) * 4u + (size_t)ngroups * 4u + 64u;
 }
 
-hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint32_t lg2s, bool adaptive,
-                              unsigned long long* counters, hipStream_t st, uint32_t* next_item, uint32_t* scratch,
-                              uint32_t resident_blocks) {
+hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
+                              bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
+                              uint32_t* scratch, uint32_t resident_blocks) {
     const uint64_t items = (uint64_t)ngroups << lg2s;
     if (items >= (1ull << 32)) return hipErrorInvalidValue;
     const uint32_t words = (uint32_t)((items + 31u) >> 5);
@@ -772,9 +791,13 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint3
     if (!adaptive) {
         k_reset_u32<<<1, 64, 0, st>>>(next_item);
         k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-        k_shadows_spl<0><<<nb, BLOCK, 0, st>>>(jobs_dev, (uint32_t)items, lg2s, next_item, bits, nullptr, nullptr,
-                                                counters);
-        k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, lg2s, bits);
+        if (njobs > 1u)
+            k_shadows_spl<0, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits,
+                                                         nullptr, nullptr, counters);
+        else
+            k_shadows_spl<0, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits,
+                                                          nullptr, nullptr, counters);
+        k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
     // scratch: refine bits (S per group at most), probe bits (4 per group), list, count
@@ -787,12 +810,22 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t ngroups, uint3
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(pwords + 255u) / 256u, 256, 0, st>>>(pbits, pwords);
     k_fill_bits<<<1, 64, 0, st>>>(count, 1u);
-    k_shadows_spl<1><<<nbp, BLOCK, 0, st>>>(jobs_dev, pitems, lg2s, next_item, pbits, nullptr, nullptr, counters);
-    k_shadow_decide<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, ngroups, 1u << lg2s, pbits, list, count);
+    if (njobs > 1u)
+        k_shadows_spl<1, true><<<nbp, BLOCK, 0, st>>>(jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr,
+                                                      nullptr, counters);
+    else
+        k_shadows_spl<1, false><<<nbp, BLOCK, 0, st>>>(jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr,
+                                                       nullptr, counters);
+    k_shadow_decide<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, pbits, list, count);
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-    k_shadows_spl<2><<<nb, BLOCK, 0, st>>>(jobs_dev, 0u, lg2s, next_item, bits, list, count, counters);
-    k_shadow_refine_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, lg2s, bits, list, count);
+    if (njobs > 1u)
+        k_shadows_spl<2, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count,
+                                                     counters);
+    else
+        k_shadows_spl<2, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count,
+                                                      counters);
+    k_shadow_refine_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, lg2s, bits, list, count);
     return hipGetLastError();
 }
 
