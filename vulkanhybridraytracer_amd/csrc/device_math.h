@@ -166,6 +166,23 @@ HR_DEV Pcg pcg_init(uint64_t initstate, uint64_t initseq) {
     return g;
 }
 HR_DEV float u01(Pcg& g) { return (float)(pcg_next(g) >> 8) * (1.0f / 16777216.0f); }
+// the state `delta` steps ahead, in O(log delta) (F. Brown, "Random number
+// generation with arbitrary strides", 1994): composes state' = a * state + c
+// by repeated squaring of (a, c)
+HR_DEV void pcg_advance(Pcg& g, uint32_t delta) {
+    uint64_t cur_mult = 6364136223846793005ULL, cur_plus = g.inc;
+    uint64_t acc_mult = 1u, acc_plus = 0u;
+    while (delta > 0u) {
+        if (delta & 1u) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1u) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1;
+    }
+    g.state = acc_mult * g.state + acc_plus;
+}
 
 // stream purposes: stream id = (purpose << 32) | index
 enum : uint32_t {

@@ -866,9 +866,17 @@ hr_status hr_render_rt(hr_scene* s, const hr_light* lights, const hr_light_shape
     if (f.max_depth == 0) return fail(ctx, HR_E_INVALID, "max_depth must be at least 1");
     if (f.adaptive) return fail(ctx, HR_E_UNSUPPORTED, "adaptive sampling is a hybrid-pass option");
     HR_TRY(check_textures(ctx, s));
+    const bool count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    const size_t wbytes = count ? 0 : rt_work_bytes(f);
+    if (wbytes) HR_TRY(ensure_work(ctx, wbytes));
     HR_TRY(frame_begin(ctx, stats != nullptr));
-    HR_HIP(ctx, launch_rt(dev_scene(s), f, reinterpret_cast<float4*>(accum), (int32_t)(int)cam->frame[0],
-                          ctx->d_counters, params->flags & HR_FLAG_COUNT_VISITS, ctx->stream));
+    if (wbytes)
+        HR_HIP(ctx, launch_rt_wavefront(dev_scene(s), f, reinterpret_cast<float4*>(accum),
+                                        (int32_t)(int)cam->frame[0], ctx->d_counters, ctx->stream, ctx->work,
+                                        ctx->d_next_group, ctx->resident_blocks));
+    else
+        HR_HIP(ctx, launch_rt(dev_scene(s), f, reinterpret_cast<float4*>(accum), (int32_t)(int)cam->frame[0],
+                              ctx->d_counters, count, ctx->stream));
     return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 

@@ -175,6 +175,12 @@ hipError_t launch_trace_rays(const DevScene& s, const hr_ray* rays, uint32_t n, 
                              unsigned long long* counters, bool count, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st);
 // pure ray tracing mode (rt.hip): accum float4[H][W], progressive when frame_signed >= 0
+// Wavefront pure ray tracing (1..64 shadow samples): workspace bytes, 0 when
+// the frame does not fit it (then launch_rt).
+size_t rt_work_bytes(const FrameArgs& f);
+hipError_t launch_rt_wavefront(const DevScene& s, const FrameArgs& f, float4* accum, int32_t frame_signed,
+                               unsigned long long* counters, hipStream_t st, void* work, uint32_t* next_item,
+                               uint32_t resident_blocks);
 hipError_t launch_rt(const DevScene& s, const FrameArgs& f, float4* accum, int32_t frame_signed,
                      unsigned long long* counters, bool count, hipStream_t st);
 

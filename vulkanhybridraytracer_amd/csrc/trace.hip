@@ -506,182 +506,82 @@ __device__ __forceinline__ uint32_t job_of(const ShadowJob* __restrict__ jobs, u
     return lo;
 }
 
+// Ray source of the hybrid shadow items (MODE and MULTI as above).
+template <int MODE, bool MULTI>
+struct HybridItems {
+    const ShadowJob* __restrict__ jobs;
+    uint32_t njobs, ilg;
+    const uint32_t* __restrict__ list;
+    const TNode4* nodes4;
+    const STri* tris;
+    uint32_t node_count;
+
+    __device__ __forceinline__ int setup(uint32_t item, RayP& r, float& tmax, const TNode4*& n4, const STri*& tr,
+                                         uint32_t& nc, Cnt& c) const {
+        const uint32_t G = MODE == 2 ? list[item >> ilg] : item >> ilg;  // global group
+        const uint32_t s = item & ((1u << ilg) - 1u);
+        const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
+        const FrameArgs& F = J.F;
+        const uint32_t SS = F.samples;
+        if (MULTI && MODE != 1 && s >= SS) return ITEM_SKIP;  // beyond this frame's sample count
+        const uint32_t g = G - J.group_base;
+        const uint32_t per_tile = F.nl * 64u;
+        const uint32_t t = g / per_tile, rem = g - t * per_tile;
+        const uint32_t li = rem >> 6, q = rem & 63u;
+        const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+        const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+        if (x >= F.width || y >= F.height) return ITEM_SKIP;
+        const bool hard = MODE == 1 && hard_light(F, li);
+        if (hard && s > 0u) return ITEM_SKIP;  // a hard light takes one probe
+        const size_t p = (size_t)y * F.width + x;
+        const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
+        const bool background = P4.w == 0.0f && N4.w == 0.0f;
+        if (background) return ITEM_SKIP;  // factor 0: no bit
+        const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
+        if (mode == 4) return ITEM_LIT;  // glass: factor 1
+        const v3 P = ld3(P4);
+        const v3 N = normalize(ld3(N4));
+        const hr_light& lt = F.lights[li];
+        const bool isDir = lt.pos_maxdist[3] < 0.0f;
+        v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
+        const float dist = length(L);
+        L = normalize(L);
+        const float NdotL = clamp01(dot(N, L));
+        if (!(NdotL > 0.0f)) return ITEM_SKIP;  // factor 0
+        if (!(isDir || dist < lt.pos_maxdist[3])) return ITEM_LIT;  // beyond the light's reach: factor 1
+        const uint32_t seed = tea(y * F.width + x, F.frame);
+        v3 dir;
+        if (MODE == 0) {
+            Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
+            dir = light_sample_dir(F, li, P, L, gs, s, SS);
+        } else {  // adaptive: probes at index li * 128 + s, refine samples at li * 128 + 64 + s
+            Pcg gs = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (MODE == 2 ? 64u + s : s));
+            dir = light_sample_dir_strat(F, li, P, L, gs, s, MODE == 2 ? SS : (hard ? 1u : 4u));
+        }
+        tmax = isDir ? 1000.0f : dist + 1.0f;
+        r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
+        ++c.shadow;
+        if (MULTI) {  // one frame: the scene stays uniform across the wave
+            n4 = J.S.nodes4;
+            tr = J.S.tris;
+            nc = J.S.node_count;
+        }
+        return ITEM_TRACE;
+    }
+};
+
 template <int MODE, bool MULTI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
     uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
     unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
-    uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
-    const uint32_t lane = threadIdx.x & 63u;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
-    uint32_t emptied = 0;
-    const TNode4* nodes4 = jobs[0].S.nodes4;
-    const STri* tris = jobs[0].S.tris;
-    uint32_t node_count = jobs[0].S.node_count;
     if (MODE == 2) nitems = *list_count << lg2s;
-    const uint32_t ilg = MODE == 1 ? 2u : lg2s;  // items per group (log2)
-    uint32_t item = 0;
-    bool tracing = false, hit = false;
-    uint32_t cur = DONE, leaf = 0u;
-    RayP r;
-    float tmax = 0.0f;
-    const float tmin = 0.001f;
-    int sp = 0;
-    uint32_t spill[MAX_STACK4 - STACK_LDS];
-    for (;;) {
-        if (cur == DONE && leaf == 0u) {
-            if (tracing) {  // the ray of `item` ended
-                if (hit) ++c.occluded;
-                else atomicOr(bits + (item >> 5), 1u << (item & 31u));
-                tracing = false;
-            }
-            bool exhausted = false;
-            for (;;) {
-                uint32_t k = 0;
-                for (uint32_t b = 1; b <= SHADOW_QUEUES; ++b)
-                    if (__ballot(emptied >= b)) k = b;
-                emptied = k;
-                if (k >= SHADOW_QUEUES) {
-                    exhausted = true;
-                    break;
-                }
-                const uint32_t band = (home + k) & (SHADOW_QUEUES - 1u);
-                const uint32_t band_end = (uint32_t)(((uint64_t)nitems * (band + 1u)) / SHADOW_QUEUES);
-                unsigned long long need = __ballot(1);
-                uint32_t rank = (uint32_t)__popcll(need & lt_mask);
-                uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(next_item + band, (uint32_t)__popcll(need));
-                base = (uint32_t)__shfl((int)base, (int)leader, 64);
-                item = (uint32_t)(((uint64_t)nitems * band) / SHADOW_QUEUES) + base + rank;
-                if (item >= band_end) {
-                    emptied = k + 1u;
-                    continue;
-                }
-                const uint32_t G = MODE == 2 ? list[item >> ilg] : item >> ilg;  // global group
-                const uint32_t s = item & ((1u << ilg) - 1u);
-                const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
-                const FrameArgs& F = J.F;
-                const uint32_t SS = F.samples;
-                if (MULTI && MODE != 1 && s >= SS) continue;  // beyond this frame's sample count
-                const uint32_t g = G - J.group_base;
-                const uint32_t per_tile = F.nl * 64u;
-                const uint32_t t = g / per_tile, rem = g - t * per_tile;
-                const uint32_t li = rem >> 6, q = rem & 63u;
-                const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-                const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
-                if (x >= F.width || y >= F.height) continue;
-                const bool hard = MODE == 1 && hard_light(F, li);
-                if (hard && s > 0u) continue;  // a hard light takes one probe
-                const size_t p = (size_t)y * F.width + x;
-                const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
-                const bool background = P4.w == 0.0f && N4.w == 0.0f;
-                if (background) continue;  // factor 0: no bit
-                const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
-                if (mode == 4) {  // glass: factor 1
-                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
-                    continue;
-                }
-                const v3 P = ld3(P4);
-                const v3 N = normalize(ld3(N4));
-                const hr_light& lt = F.lights[li];
-                const bool isDir = lt.pos_maxdist[3] < 0.0f;
-                v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
-                const float dist = length(L);
-                L = normalize(L);
-                const float NdotL = clamp01(dot(N, L));
-                if (!(NdotL > 0.0f)) continue;  // factor 0
-                if (!(isDir || dist < lt.pos_maxdist[3])) {  // beyond the light's reach: factor 1
-                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
-                    continue;
-                }
-                const uint32_t seed = tea(y * F.width + x, F.frame);
-                v3 dir;
-                if (MODE == 0) {
-                    Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
-                    dir = light_sample_dir(F, li, P, L, gs, s, SS);
-                } else {  // adaptive: probes at index li * 128 + s, refine samples at li * 128 + 64 + s
-                    Pcg gs = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (MODE == 2 ? 64u + s : s));
-                    dir = light_sample_dir_strat(F, li, P, L, gs, s, MODE == 2 ? SS : (hard ? 1u : 4u));
-                }
-                tmax = isDir ? 1000.0f : dist + 1.0f;
-                r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
-                ++c.shadow;
-                if (MULTI) {  // one frame: the scene stays uniform across the wave
-                    nodes4 = J.S.nodes4;
-                    tris = J.S.tris;
-                    node_count = J.S.node_count;
-                }
-                tracing = true;
-                hit = false;
-                sp = 0;
-                cur = node_count ? 0u : DONE;
-                break;
-            }
-            if (exhausted) break;
-            continue;
-        }
-        while (cur < LEAF_BIT) {
-            cur = wide_step<MAX_STACK4>(nodes4, cur, r, tmin, tmax, lds, spill, sp, c);
-            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
-                leaf = cur;
-                if (sp == 0) {
-                    cur = DONE;
-                } else {
-                    --sp;
-                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
-                }
-            }
-            if (__ballot(leaf == 0u) == 0ull) break;
-        }
-        uint32_t lf = DONE;
-        bool in_order = false;
-        if (leaf != 0u) {
-            lf = leaf;
-            leaf = 0u;
-        } else if (cur >= LEAF_BIT && cur != DONE) {
-            lf = cur;
-            in_order = true;
-        }
-        if (lf != DONE) {
-            const uint32_t first = lf & 0x07FFFFFFu;
-            const uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
-            for (uint32_t k = first; k < first + cnt; ++k) {
-                const gf4 tp = gptr(tris + k);
-                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-                float t, b1, b2;
-                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2)) {
-                    hit = true;
-                    break;
-                }
-            }
-            if (hit) {
-                cur = DONE;
-                leaf = 0u;
-            } else if (in_order) {
-                if (sp == 0) {
-                    cur = DONE;
-                } else {
-                    --sp;
-                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
-                }
-            }
-        }
-    }
+    HybridItems<MODE, MULTI> src{jobs, njobs, MODE == 1 ? 2u : lg2s, list, jobs[0].S.nodes4, jobs[0].S.tris,
+                                 jobs[0].S.node_count};
+    item_loop(src, nitems, next_item, bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
-}
-
-// the number of set bits of a group's item window (stride 2^lg2s, 2 <= lg2s <= 6):
-// whole words for 32 and 64, a masked part of one word below (a shift by 32
-// would not give a full mask)
-__device__ __forceinline__ uint32_t window_bits(const uint32_t* __restrict__ bits, uint64_t b0, uint32_t lg2s) {
-    if (lg2s == 6u) return (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
-    if (lg2s == 5u) return (uint32_t)__popc(bits[b0 >> 5]);
-    const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
-    return (uint32_t)__popc(w & ((1u << (1u << lg2s)) - 1u));
 }
 
 // shadow factor of every group from the bits of its samples

@@ -643,5 +643,138 @@ __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, 
     return g;
 }
 
+// ---- persistent any-hit tracing over numbered work items ----------------------
+// Items 0..nitems-1 are handed out from SHADOW_QUEUES bands (one atomic counter
+// each in next_item; workgroup b starts at band b mod SHADOW_QUEUES and moves
+// to the next band when its own runs dry). Lanes that take work together take
+// consecutive items. A lane whose ray ended takes its next item at once, while
+// the rest of the wave keeps traversing. src.setup(item, ...) decides an item:
+// ITEM_SKIP (no bit), ITEM_LIT (bit set without a ray) or ITEM_TRACE (ray,
+// tmax and scene set); an unoccluded traced item sets bit `item` of `bits`
+// (atomicOr, so the result does not depend on the order).
+enum : int { ITEM_SKIP = 0, ITEM_LIT = 1, ITEM_TRACE = 2 };
+
+template <class Src>
+__device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* next_item, uint32_t* bits,
+                                          uint32_t* lds, Cnt& c) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
+    uint32_t emptied = 0;
+    const TNode4* nodes4 = src.nodes4;
+    const STri* tris = src.tris;
+    uint32_t node_count = src.node_count;
+    uint32_t item = 0;
+    bool tracing = false, hit = false;
+    uint32_t cur = DONE, leaf = 0u;
+    RayP r;
+    float tmax = 0.0f;
+    const float tmin = 0.001f;
+    int sp = 0;
+    uint32_t spill[MAX_STACK4 - STACK_LDS];
+    for (;;) {
+        if (cur == DONE && leaf == 0u) {
+            if (tracing) {  // the ray of `item` ended
+                if (hit) ++c.occluded;
+                else atomicOr(bits + (item >> 5), 1u << (item & 31u));
+                tracing = false;
+            }
+            bool exhausted = false;
+            for (;;) {
+                uint32_t k = 0;
+                for (uint32_t b = 1; b <= SHADOW_QUEUES; ++b)
+                    if (__ballot(emptied >= b)) k = b;
+                emptied = k;
+                if (k >= SHADOW_QUEUES) {
+                    exhausted = true;
+                    break;
+                }
+                const uint32_t band = (home + k) & (SHADOW_QUEUES - 1u);
+                const uint32_t band_end = (uint32_t)(((uint64_t)nitems * (band + 1u)) / SHADOW_QUEUES);
+                unsigned long long need = __ballot(1);
+                uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+                uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(next_item + band, (uint32_t)__popcll(need));
+                base = (uint32_t)__shfl((int)base, (int)leader, 64);
+                item = (uint32_t)(((uint64_t)nitems * band) / SHADOW_QUEUES) + base + rank;
+                if (item >= band_end) {
+                    emptied = k + 1u;
+                    continue;
+                }
+                const int kind = src.setup(item, r, tmax, nodes4, tris, node_count, c);
+                if (kind == ITEM_SKIP) continue;
+                if (kind == ITEM_LIT) {
+                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
+                    continue;
+                }
+                tracing = true;
+                hit = false;
+                sp = 0;
+                cur = node_count ? 0u : DONE;
+                break;
+            }
+            if (exhausted) break;
+            continue;
+        }
+        while (cur < LEAF_BIT) {
+            cur = wide_step<MAX_STACK4>(nodes4, cur, r, tmin, tmax, lds, spill, sp, c);
+            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+                leaf = cur;
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+            if (__ballot(leaf == 0u) == 0ull) break;
+        }
+        uint32_t lf = DONE;
+        bool in_order = false;
+        if (leaf != 0u) {
+            lf = leaf;
+            leaf = 0u;
+        } else if (cur >= LEAF_BIT && cur != DONE) {
+            lf = cur;
+            in_order = true;
+        }
+        if (lf != DONE) {
+            const uint32_t first = lf & 0x07FFFFFFu;
+            const uint32_t cnt = ((lf >> 27) & 0xFu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                const gf4 tp = gptr(tris + k);
+                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, b1, b2;
+                if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2)) {
+                    hit = true;
+                    break;
+                }
+            }
+            if (hit) {
+                cur = DONE;
+                leaf = 0u;
+            } else if (in_order) {
+                if (sp == 0) {
+                    cur = DONE;
+                } else {
+                    --sp;
+                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                }
+            }
+        }
+    }
+}
+
+// the number of set bits of a group's item window (stride 2^lg2s, lg2s <= 6):
+// whole words for 32 and 64, a masked part of one word below (a shift by 32
+// would not give a full mask)
+__device__ __forceinline__ uint32_t window_bits(const uint32_t* __restrict__ bits, uint64_t b0, uint32_t lg2s) {
+    if (lg2s == 6u) return (uint32_t)__popc(bits[b0 >> 5]) + (uint32_t)__popc(bits[(b0 >> 5) + 1u]);
+    if (lg2s == 5u) return (uint32_t)__popc(bits[b0 >> 5]);
+    const uint32_t w = bits[b0 >> 5] >> (uint32_t)(b0 & 31u);
+    return (uint32_t)__popc(w & ((1u << (1u << lg2s)) - 1u));
+}
+
 }  // namespace tc
 }  // namespace hr
