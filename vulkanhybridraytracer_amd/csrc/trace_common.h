@@ -50,6 +50,13 @@ __device__ __forceinline__ void sort_pair(float& ka, uint32_t& ca, float& kb, ui
 }
 
 constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (fewer than 2^27 triangles)
+// The inner traversal loops also end when fewer lanes than this would keep
+// walking, so lanes whose ray ended or that hold two leaves get back to work
+// (leaf tests, new rays) instead of idling; results do not depend on it.
+// consume_rays (queues of closest-hit and hit-shadow rays) and item_loop
+// (sample-parallel shadow items) have their own.
+constexpr int REFILL_MIN_LANES = 16;
+constexpr int REFILL_MIN_ITEMS = 32;
 
 // Node and triangle arrays live in global memory; reading them through
 // global-address-space pointers gives global (not flat) loads, whose waits do
@@ -471,7 +478,7 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
                     cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
                 }
             }
-            if (__ballot(leaf == 0u) == 0ull) break;
+            if (__ballot(leaf == 0u) == 0ull || __popcll(__ballot(cur < LEAF_BIT)) < REFILL_MIN_LANES) break;
         }
         while (COUNT && cur < LEAF_BIT) {
             const float4* np = reinterpret_cast<const float4*>(S.nodes + cur);
@@ -728,7 +735,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                     cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
                 }
             }
-            if (__ballot(leaf == 0u) == 0ull) break;
+            if (__ballot(leaf == 0u) == 0ull || __popcll(__ballot(cur < LEAF_BIT)) < REFILL_MIN_ITEMS) break;
         }
         uint32_t lf = DONE;
         bool in_order = false;
