@@ -467,9 +467,10 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
         }
         while (!COUNT && cur < LEAF_BIT) {
             cur = wide_step<STACK_CAP>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
-            // postpone the first leaf found and keep walking until every lane
-            // still in this loop holds one
-            if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
+            // any hit: postpone the first leaf found and keep walking until
+            // every lane still in this loop holds one. Closest hit: leaves in
+            // walk order, so each one can shrink tbest before the next boxes
+            if (ANY && leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
                 leaf = cur;
                 if (sp == 0) {
                     cur = DONE;
