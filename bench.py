@@ -47,6 +47,16 @@ def copy_probe(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def shadow_kernel_name(params):
+    """The kernel(s) of the timed shadow pass: sample-parallel for 4..64
+    samples (a power of two), in two phases with adaptive sampling."""
+    if params["shadow_samples"] not in (4, 8, 16, 32, 64):
+        return "k_shadows"
+    if params.get("flags", 0) & 4:
+        return "k_shadows_spl (probe and refine phases, with k_shadow_decide)"
+    return "k_shadows_spl"
+
+
 def load_traffic(workload):
     """HBM bytes per launch of k_shadows measured by rocprofv3 --pmc runs and
     committed under profiles/ (None when absent)."""
@@ -300,8 +310,7 @@ def main():
     traffic, traffic_src = load_traffic(sc.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "k_shadows_spl" if (not sc.params.get("flags", 0) & 4 and
-                                              sc.params["shadow_samples"] in (4, 8, 16, 32, 64)) else "k_shadows",
+                "kernel": shadow_kernel_name(sc.params),
                 "algorithmic_bytes": int(bytes_shadow),
                 "avg_launch_ms": round(avg_shadow_ms, 3),
                 "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),

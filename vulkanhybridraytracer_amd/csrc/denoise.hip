@@ -50,14 +50,14 @@ constexpr int CONV_NT = 4;  // 16-channel tiles per wave
 
 // Epilogue of both convolution kernels: bias, ReLU, and 4 channels per lane
 // stored into the output's channel slice (BF16 8-byte or FP32 16-byte writes).
-__device__ __forceinline__ void conv_store(const ConvArgs& a, const f32x4 (&acc)[CONV_MT][CONV_NT],
+template <int NS>
+__device__ __forceinline__ void conv_store(const ConvArgs& a, const f32x4 (&acc)[CONV_MT][NS],
                                            const uint32_t (&pix)[CONV_MT], const bool (&pok)[CONV_MT],
-                                           uint32_t s0, uint32_t nsub, uint32_t lane) {
+                                           uint32_t s0, uint32_t lane) {
     // C layout: row (channel) = (lane / 16) * 4 + r, column (pixel) = lane % 16
     const bool vec_ok = ((a.out_C | a.out_off) & 3u) == 0;  // 4-channel groups aligned
 #pragma unroll
-    for (int s = 0; s < CONV_NT; ++s) {
-        if ((uint32_t)s >= nsub) continue;
+    for (int s = 0; s < NS; ++s) {
         const uint32_t co = (s0 + s) * 16u + (lane >> 4) * 4u;
         if (co >= a.Cout) continue;
         float b[4];
@@ -94,20 +94,22 @@ __device__ __forceinline__ void conv_store(const ConvArgs& a, const f32x4 (&acc)
     }
 }
 
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+// NS 16-channel output tiles per wave (a compile-time count, so the
+// accumulators stay in AGPRs): tiles s_base + blockIdx.y * NS onwards.
+template <int NS>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t s_base) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t P = a.N * a.H * a.W;
     const uint32_t pix0 = wave * (16u * CONV_MT);
     if (pix0 >= P) return;
-    const uint32_t s0 = blockIdx.y * CONV_NT;          // first 16-channel tile
+    const uint32_t s0 = s_base + blockIdx.y * NS;  // first 16-channel tile
     const uint32_t ns_total = a.Cout_pad >> 4;
-    const uint32_t nsub = min((uint32_t)CONV_NT, ns_total - s0);
-    f32x4 acc[CONV_MT][CONV_NT];
+    f32x4 acc[CONV_MT][NS];
 #pragma unroll
     for (int m = 0; m < CONV_MT; ++m)
 #pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < NS; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
     int py[CONV_MT], px[CONV_MT];
     long pbase[CONV_MT];
     bool pv[CONV_MT];
@@ -142,13 +144,11 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         }
         const bf16x8* wk = wfrag + (size_t)(kk >> 5) * wstep;
 #pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) {
-            if ((uint32_t)s < nsub) {
-                const bf16x8 bv = wk[s * 64];
+        for (int s = 0; s < NS; ++s) {
+            const bf16x8 bv = wk[s * 64];
 #pragma unroll
-                for (int m = 0; m < CONV_MT; ++m)
-                    acc[m][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv, av[m], acc[m][s], 0, 0, 0);
-            }
+            for (int m = 0; m < CONV_MT; ++m)
+                acc[m][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv, av[m], acc[m][s], 0, 0, 0);
         }
         ci += 32u;
         while (ci >= a.Cin && tap < 9u) {
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         pix[m] = pix0 + m * 16u + (lane & 15u);
         pok[m] = pix[m] < P;
     }
-    conv_store(a, acc, pix, pok, s0, nsub, lane);
+    conv_store<NS>(a, acc, pix, pok, s0, lane);
 }
 
 // The same convolution with the input staged in LDS: a workgroup computes an
@@ -173,8 +173,10 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 // fragments from LDS instead of re-reading global memory. Same K walk,
 // fragments and MFMA order per K step as k_conv.
 constexpr int LT_H = 8, LT_W = 32, HALO_H = LT_H + 2, HALO_W = LT_W + 2;
+constexpr int HALO_BATCH = 8;
 
-__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y) {
+template <int NS>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t s_base) {
     extern __shared__ __align__(16) __bf16 tile[];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint32_t ps = a.Cin + 8u;
@@ -188,24 +190,37 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     const uint32_t cpp = a.Cin >> 3;  // 16-byte chunks per pixel
     const uint32_t total = (uint32_t)(HALO_H * HALO_W) * cpp;
     const bf16x8 zero = {};
-    for (uint32_t i = threadIdx.x; i < total; i += 256u) {
-        const uint32_t hp = i / cpp, ch = i - hp * cpp;
-        const int y = y0 + (int)(hp / HALO_W) - 1, x = x0 + (int)(hp % HALO_W) - 1;
-        bf16x8 v = zero;
-        if (y >= 0 && y < H && x >= 0 && x < W)
-            v = *reinterpret_cast<const bf16x8*>(a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C +
-                                                 a.in_off + ch * 8u);
-        *reinterpret_cast<bf16x8*>(tile + (size_t)hp * ps + ch * 8u) = v;
+    // HALO_BATCH chunks per thread in flight before they are written to LDS,
+    // so the global latency is paid once per batch, not once per chunk
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += 256u * HALO_BATCH) {
+        bf16x8 v[HALO_BATCH];
+        uint32_t dst[HALO_BATCH];
+#pragma unroll
+        for (int j = 0; j < HALO_BATCH; ++j) {
+            const uint32_t i = i0 + (uint32_t)j * 256u;
+            v[j] = zero;
+            dst[j] = ~0u;
+            if (i < total) {
+                const uint32_t hp = i / cpp, ch = i - hp * cpp;
+                const int y = y0 + (int)(hp / HALO_W) - 1, x = x0 + (int)(hp % HALO_W) - 1;
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    v[j] = *reinterpret_cast<const bf16x8*>(
+                        a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C + a.in_off + ch * 8u);
+                dst[j] = hp * ps + ch * 8u;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < HALO_BATCH; ++j)
+            if (dst[j] != ~0u) *reinterpret_cast<bf16x8*>(tile + dst[j]) = v[j];
     }
     __syncthreads();
-    const uint32_t s0 = blockIdx.y * CONV_NT;
+    const uint32_t s0 = s_base + blockIdx.y * NS;
     const uint32_t ns_total = a.Cout_pad >> 4;
-    const uint32_t nsub = min((uint32_t)CONV_NT, ns_total - s0);
-    f32x4 acc[CONV_MT][CONV_NT];
+    f32x4 acc[CONV_MT][NS];
 #pragma unroll
     for (int m = 0; m < CONV_MT; ++m)
 #pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < NS; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
     int rbase[CONV_MT];  // LDS pixel index of this lane's pixel for tap (0, 0) minus one row and column
     uint32_t pix[CONV_MT];
     bool pok[CONV_MT];
@@ -224,12 +239,10 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     }
     const bf16x8* wfrag = reinterpret_cast<const bf16x8*>(a.wt) + (size_t)s0 * 64u + lane;
     const size_t wstep = (size_t)ns_total * 64u;
-    // weight fragments one K step ahead, so their global latency overlaps
-    // the current step's MFMAs
-    bf16x8 wcur[CONV_NT];
-#pragma unroll
-    for (int s = 0; s < CONV_NT; ++s) wcur[s] = (uint32_t)s < nsub ? wfrag[s * 64] : zero;
-    for (uint32_t kk = 0; kk < a.Kpad; kk += 32u) {
+    const uint32_t nk = a.Kpad >> 5;  // 32-wide K steps
+    // One K step: this lane's 8 input channels of its 4 pixels from LDS, then
+    // CONV_MT x NS MFMAs with the step's weight fragments.
+    auto kstep = [&](const bf16x8 (&wv)[NS]) {
         const bool kval = tap < 9u;
         const uint32_t tc = kval ? tap : 4u;  // padding K: any in-tile address, value unused
         const int doff = ((int)(tc / 3u) - 1) * HALO_W + (int)(tc % 3u) - 1;
@@ -239,28 +252,37 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
             const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (size_t)(rbase[m] + doff) * ps + ci);
             av[m] = kval ? v : zero;
         }
-        bf16x8 wnext[CONV_NT];
-        const bool more = kk + 32u < a.Kpad;
-        const bf16x8* wk = wfrag + (size_t)((kk >> 5) + 1u) * wstep;
 #pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) wnext[s] = (more && (uint32_t)s < nsub) ? wk[s * 64] : zero;
+        for (int s = 0; s < NS; ++s)
 #pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) {
-            if ((uint32_t)s < nsub) {
-#pragma unroll
-                for (int m = 0; m < CONV_MT; ++m)
-                    acc[m][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wcur[s], av[m], acc[m][s], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < CONV_NT; ++s) wcur[s] = wnext[s];
+            for (int m = 0; m < CONV_MT; ++m)
+                acc[m][s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[s], av[m], acc[m][s], 0, 0, 0);
         ci += 32u;
         while (ci >= a.Cin && tap < 9u) {
             ci -= a.Cin;
             ++tap;
         }
+    };
+    // Weight fragments one K step ahead in two buffers used in turn (no copy
+    // between them, so waiting for one does not wait for the other). The
+    // prefetch index is clamped to the last step: always in bounds, and the
+    // value loaded past the end is not used.
+    auto wload = [&](bf16x8 (&wv)[NS], uint32_t k) {
+        const bf16x8* wk = wfrag + (size_t)min(k, nk - 1u) * wstep;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wv[s] = wk[s * 64];
+    };
+    bf16x8 wa[NS], wb[NS];
+    wload(wa, 0u);
+    uint32_t k = 0;
+    for (; k + 2u <= nk; k += 2u) {
+        wload(wb, k + 1u);
+        kstep(wa);
+        wload(wa, k + 2u);
+        kstep(wb);
     }
-    conv_store(a, acc, pix, pok, s0, nsub, lane);
+    if (k < nk) kstep(wa);  // odd step count: the last step
+    conv_store<NS>(a, acc, pix, pok, s0, lane);
 }
 
 // ---- weight gradient --------------------------------------------------------------
@@ -647,19 +669,32 @@ __global__ void k_pack_weights(PackArgs a) {
 // ---- launchers ------------------------------------------------------------------------
 static inline uint32_t nblk(size_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
-    const uint32_t ny = ((a.Cout_pad >> 4) + dn::CONV_NT - 1) / dn::CONV_NT;
-    // LDS-staged halo when the tile fits in 64 KiB and 16-byte chunks stay aligned
-    const size_t lds = (size_t)dn::HALO_H * dn::HALO_W * (a.Cin + 8u) * sizeof(__bf16);
-    if (lds <= 65536u && (a.Cin & 7u) == 0 && (a.in_C & 7u) == 0 && (a.in_off & 7u) == 0) {
+// One launch per tile count: CONV_NT tiles per wave over the first
+// (Cout_pad / 16) / CONV_NT groups, the remaining 1..3 tiles in a second launch.
+template <int NS>
+static void conv_launch(const ConvArgs& a, uint32_t ny, uint32_t s_base, bool lds_path, size_t lds, hipStream_t st) {
+    if (lds_path) {
         const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
-        dim3 grid(a.N * tx * ty, ny);
-        dn::k_conv_lds<<<grid, 256, lds, st>>>(a, tx, ty);
-        return hipGetLastError();
+        dn::k_conv_lds<NS><<<dim3(a.N * tx * ty, ny), 256, lds, st>>>(a, tx, ty, s_base);
+    } else {
+        const size_t P = (size_t)a.N * a.H * a.W;
+        dn::k_conv<NS><<<dim3(nblk(P, 4 * 16 * dn::CONV_MT), ny), 256, 0, st>>>(a, s_base);
     }
-    size_t P = (size_t)a.N * a.H * a.W;
-    dim3 grid(nblk(P, 4 * 16 * dn::CONV_MT), ny);
-    dn::k_conv<<<grid, 256, 0, st>>>(a);
+}
+
+hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
+    // LDS-staged halo when the tile fits in 96 KiB (up to 128 input channels;
+    // gfx950 has 160 KiB per CU) and 16-byte chunks stay aligned
+    const size_t lds = (size_t)dn::HALO_H * dn::HALO_W * (a.Cin + 8u) * sizeof(__bf16);
+    const bool lds_path = lds <= 98304u && (a.Cin & 7u) == 0 && (a.in_C & 7u) == 0 && (a.in_off & 7u) == 0;
+    const uint32_t ns_total = a.Cout_pad >> 4;
+    const uint32_t full = ns_total / dn::CONV_NT, rem = ns_total % dn::CONV_NT;
+    static_assert(dn::CONV_NT == 4, "remainder launches below cover 1..3 tiles");
+    if (full) conv_launch<4>(a, full, 0u, lds_path, lds, st);
+    const uint32_t sb = full * dn::CONV_NT;
+    if (rem == 1) conv_launch<1>(a, 1u, sb, lds_path, lds, st);
+    if (rem == 2) conv_launch<2>(a, 1u, sb, lds_path, lds, st);
+    if (rem == 3) conv_launch<3>(a, 1u, sb, lds_path, lds, st);
     return hipGetLastError();
 }
 uint32_t dn_wgrad_chunks(uint32_t P, uint32_t* chunk) {
