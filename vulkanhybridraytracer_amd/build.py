@@ -17,7 +17,10 @@ HEADERS = ["csrc/device_math.h", "csrc/internal.h", "csrc/trace_common.h", "csrc
 LIBS = ["-lz"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",
-         "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+         "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
+         # MFMA accumulators in VGPRs (gfx950 reads and writes them there): no
+         # copies between VGPRs and AGPRs around the denoiser's MFMAs
+         "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def hipcc():

@@ -29,6 +29,8 @@
 
 #include "internal.h"
 
+#include <type_traits>
+
 namespace hr {
 namespace dn {
 
@@ -221,13 +223,13 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     for (int m = 0; m < CONV_MT; ++m)
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[m][s] = f32x4{0.f, 0.f, 0.f, 0.f};
-    int rbase[CONV_MT];  // LDS pixel index of this lane's pixel for tap (0, 0) minus one row and column
+    int rbase[CONV_MT];  // LDS element offset of this lane's pixel (tap (1, 1) is the pixel itself)
     uint32_t pix[CONV_MT];
     bool pok[CONV_MT];
 #pragma unroll
     for (int m = 0; m < CONV_MT; ++m) {
         const int ry = (int)(2u * w) + (m >> 1), rx = (m & 1) * 16 + (int)(lane & 15u);
-        rbase[m] = (ry + 1) * HALO_W + rx + 1;
+        rbase[m] = ((ry + 1) * HALO_W + rx + 1) * (int)ps;
         const int y = y0 + ry, x = x0 + rx;
         pok[m] = y < H && x < W;
         pix[m] = pok[m] ? (n * a.H + (uint32_t)y) * a.W + (uint32_t)x : 0u;
@@ -241,15 +243,16 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     const size_t wstep = (size_t)ns_total * 64u;
     const uint32_t nk = a.Kpad >> 5;  // 32-wide K steps
     // One K step: this lane's 8 input channels of its 4 pixels from LDS, then
-    // CONV_MT x NS MFMAs with the step's weight fragments.
-    auto kstep = [&](const bf16x8 (&wv)[NS]) {
-        const bool kval = tap < 9u;
+    // CONV_MT x NS MFMAs with the step's weight fragments. Only the last step
+    // can hold padding K (Kpad - 9 Cin < 32), so only it checks the tap.
+    auto kstep = [&](const bf16x8 (&wv)[NS], auto last) {
+        const bool kval = !decltype(last)::value || tap < 9u;
         const uint32_t tc = kval ? tap : 4u;  // padding K: any in-tile address, value unused
-        const int doff = ((int)(tc / 3u) - 1) * HALO_W + (int)(tc % 3u) - 1;
+        const int doff = (((int)(tc / 3u) - 1) * HALO_W + (int)(tc % 3u) - 1) * (int)ps + (int)ci;
         bf16x8 av[CONV_MT];
 #pragma unroll
         for (int m = 0; m < CONV_MT; ++m) {
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (size_t)(rbase[m] + doff) * ps + ci);
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (rbase[m] + doff));
             av[m] = kval ? v : zero;
         }
 #pragma unroll
@@ -272,16 +275,25 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
 #pragma unroll
         for (int s = 0; s < NS; ++s) wv[s] = wk[s * 64];
     };
+    using full = std::false_type;  // every K of the step is a real tap
+    using last = std::true_type;   // the last step
     bf16x8 wa[NS], wb[NS];
     wload(wa, 0u);
+    const uint32_t nmain = nk - 1u;
     uint32_t k = 0;
-    for (; k + 2u <= nk; k += 2u) {
+    for (; k + 2u <= nmain; k += 2u) {
         wload(wb, k + 1u);
-        kstep(wa);
+        kstep(wa, full{});
         wload(wa, k + 2u);
-        kstep(wb);
+        kstep(wb, full{});
     }
-    if (k < nk) kstep(wa);  // odd step count: the last step
+    if (k < nmain) {  // one full step left before the last
+        wload(wb, k + 1u);
+        kstep(wa, full{});
+        kstep(wb, last{});
+    } else {
+        kstep(wa, last{});
+    }
     conv_store<NS>(a, acc, pix, pok, s0, lane);
 }
 
