@@ -388,7 +388,11 @@ hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);
  * accum: device float4[height][width]; camera->frame[0] < 0 stores the new
  * colour, otherwise accum = mix(accum, colour, 1 / (1 + frame))
  * (raygen.rgen:69-80). stats: primary_rays, reflection_rays (the chain's
- * later rays), hit_shadow_rays, occluded_rays, ms_shade. */
+ * later rays), hit_shadow_rays, occluded_rays, ms_shade. Without
+ * HR_FLAG_COUNT_VISITS and with at most 64 shadow samples the frame runs as
+ * wavefront kernels in the context's workspace (about 128 B per pixel plus
+ * light_count * 2^ceil(log2 samples) bits per pixel); otherwise one lane per
+ * pixel walks the whole chain. Both give the same image. */
 hr_status hr_render_rt(hr_scene* scene, const hr_light* lights, const hr_light_shape* shapes, uint32_t light_count,
                        const hr_camera* camera, const hr_trace_params* params, uint32_t width, uint32_t height,
                        float* accum, hr_stats* stats);
