@@ -242,7 +242,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the hybrid pass is built and measured on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(frames would shard by screen tiles, batches by scene); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "hybrid_batch32":
