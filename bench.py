@@ -58,13 +58,16 @@ def shadow_kernel_name(params):
 
 
 def load_traffic(workload):
-    """HBM bytes per launch of k_shadows measured by rocprofv3 --pmc runs and
-    committed under profiles/ (None when absent)."""
+    """HBM bytes per launch of the shadow kernel measured by rocprofv3 --pmc
+    runs and committed under profiles/, with the L2 hit rate, lane
+    utilisation, VALU busy and occupancy of the same runs (None when absent)."""
     p = os.path.join(ROOT, "profiles", "k_shadows_%s_traffic.json" % workload)
     if not os.path.exists(p):
-        return None, None
+        return None, None, {}
     d = json.load(open(p))
-    return d.get("hbm_bytes_per_launch"), os.path.relpath(p, ROOT)
+    counters = {k: d[k] for k in ("l2_hit_rate", "valu_utilization", "valu_busy", "occupancy_waves_per_simd",
+                                  "achieved_hbm_gbs") if k in d}
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(p, ROOT), counters
 
 
 def batch_main(args):
@@ -307,7 +310,7 @@ def main():
     bytes_shadow = shadow_nodes * 64 + shadow_tris * 48 + W * H * (32 + 4 * nl)
     avg_shadow_ms = float(np.mean(ms_shadow))
     achieved = bytes_shadow / (avg_shadow_ms * 1e-3) / 1e9
-    traffic, traffic_src = load_traffic(sc.name)
+    traffic, traffic_src, counters = load_traffic(sc.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": shadow_kernel_name(sc.params),
@@ -315,6 +318,8 @@ def main():
                 "avg_launch_ms": round(avg_shadow_ms, 3),
                 "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),
                 "tris_per_ray": round(shadow_tris / max(shadow_rays, 1), 2)}
+    if counters:
+        roofline["rocprof"] = counters  # measured HBM GB/s, L2 hit rate, VALU, occupancy (profiles/)
 
     cpu = None
     if not args.no_cpu_baseline:
