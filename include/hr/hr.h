@@ -299,7 +299,11 @@ hr_status hr_scene_info(const hr_scene* scene, uint32_t* tri_count, uint32_t* no
  * triangles (tri_count x 48 B), 2 = Morton codes in sorted order (tri_count x
  * 4 B), 3 = sorted triangle ids (tri_count x 4 B), 4 = Karras internal nodes
  * (left, right, first, last) ((tri_count - 1) x 16 B), 5 = 4-wide traversal
- * nodes (node_count x 128 B, slots of binary nodes at even depth). */
+ * nodes (node_count x 128 B; the slot of each wide root, a binary node, holds
+ * its 2-4 children: from the root down, a wide node takes its binary node's
+ * two children and twice replaces the internal child of largest box area,
+ * (x y + y z) + z x of the extents, first on ties, by that child's two
+ * children). */
 hr_status hr_scene_export(const hr_scene* scene, uint32_t which, void* dst_host, size_t bytes);
 
 /* ---- per-frame passes ---------------------------------------------------- */

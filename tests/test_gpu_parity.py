@@ -239,7 +239,8 @@ def _binary_children(nodes, i):
 def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
     """the 4-wide nodes of the timed kernels: from the root they reach exactly
     the binary tree's leaves, each child box equal to the binary box of the
-    same subtree, empty slots with inverted boxes"""
+    same subtree, empty slots with inverted boxes, and every wide node is the
+    greedy largest-area collapse of its binary node"""
     sc = scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=24, soup_tris=256)
     hr = _hr()
     hs = hr.HybridScene(gpu_ctx, sc, leaf_max=leaf_max)
@@ -280,3 +281,38 @@ def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
                 stack.append(code)
     assert sorted(bin_leaves) == sorted(wide_leaves)
     assert len(wide_leaves) > 100
+
+    # the collapse rule itself: from the binary node's two children, twice open
+    # the internal child of largest area (x y + y z) + z x of the extents in
+    # float32, the first one on ties; the opened child's first child takes its
+    # slot and the second is appended
+    f32 = np.float32
+
+    def area(lo, hi):
+        e = [max(f32(hi[a]) - f32(lo[a]), f32(0)) for a in range(3)]
+        return (e[0] * e[1] + e[1] * e[2]) + e[2] * e[0]
+
+    stack, roots = [0], 0
+    while stack:
+        i = stack.pop()
+        roots += 1
+        ch = _binary_children(bn, i)
+        for _ in range(2):
+            best, best_a = -1, f32(-1)
+            for k, (code, lo, hi) in enumerate(ch):
+                if not code & LEAF and area(lo, hi) > best_a:
+                    best, best_a = k, area(lo, hi)
+            if best < 0:
+                break
+            g = _binary_children(bn, ch[best][0])
+            ch[best] = g[0]
+            ch.append(g[1])
+        got = wide(i)
+        for k in range(4):
+            if k < len(ch):
+                assert got[k][0] == ch[k][0], (i, k)
+                assert np.array_equal(got[k][1], ch[k][1]) and np.array_equal(got[k][2], ch[k][2]), (i, k)
+            else:
+                assert got[k][0] == 0xFFFFFFFF, (i, k)
+        stack += [c for c, _, _ in ch if not c & LEAF]
+    assert roots > 10

@@ -28,13 +28,15 @@ __host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) { 
 struct __align__(64) TNode {
     float4 a, b, c, d;
 };
-// 4-wide traversal node, 128 B, one per binary node at even depth (slot =
-// the binary node's index): child k's box in component k of lo/hi x/y/z and
-// its code in code[k] (an internal slot, a leaf code, or EMPTY4 with an
-// inverted box). Built from the binary nodes by taking grandchildren; the
+// 4-wide traversal node, 128 B, one per wide root (slot = the binary node's
+// index): child k's box in component k of lo/hi x/y/z and its code in code[k]
+// (an internal slot, a leaf code, or EMPTY4 with an inverted box). Built from
+// the binary nodes top-down: a wide node starts from its binary node's two
+// children and twice opens the internal child with the largest box area. The
 // timed traversal kernels walk it, the counting kernels walk the binary one.
 constexpr uint32_t EMPTY4 = 0xFFFFFFFFu;
-constexpr uint32_t MAX_STACK4 = 3u * (MAX_DEPTH / 2u) + 1u;  // 3 pushes per 4-wide level
+// 3 pushes per 4-wide level; a wide level descends at least one binary level
+constexpr uint32_t MAX_STACK4 = 3u * MAX_DEPTH + 1u;
 struct __align__(64) TNode4 {
     float4 lox, hix, loy, hiy, loz, hiz;
     uint4 code;
@@ -230,7 +232,7 @@ struct BuildBuffers {
     uint32_t* flags;    // refit arrival counters
     float4* kbox;       // internal node boxes x 2
     TNode* nodes;
-    TNode4* nodes4;     // node_count slots, filled at even depths
+    TNode4* nodes4;     // node_count slots, filled at the wide roots
     STri* stris;
     uint32_t* tri_prim;
     uint32_t* depth;    // 1 uint
