@@ -611,6 +611,7 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     check(scene_alloc(s, &b.kparent, nn));
     check(scene_alloc(s, &b.lparent, n));
     check(scene_alloc(s, &b.flags, nn));
+    check(scene_alloc(s, &b.lvl, nn + 2 * (size_t)REFIT_LEVELS + 1));
     check(scene_alloc(s, &b.kbox, nn * 2));
     check(scene_alloc(s, &b.nodes, nn));
     check(scene_alloc(s, &b.nodes4, nn));

@@ -35,6 +35,9 @@ struct __align__(64) TNode {
 // children and twice opens the internal child with the largest box area. The
 // timed traversal kernels walk it, the counting kernels walk the binary one.
 constexpr uint32_t EMPTY4 = 0xFFFFFFFFu;
+// depth buckets of the level-by-level box refit (a tree that passes the depth
+// check has internal nodes at most MAX_DEPTH + 16 edges deep)
+constexpr uint32_t REFIT_LEVELS = 128;
 // 3 pushes per 4-wide level; a wide level descends at least one binary level
 constexpr uint32_t MAX_STACK4 = 3u * MAX_DEPTH + 1u;
 struct __align__(64) TNode4 {
@@ -229,7 +232,10 @@ struct BuildBuffers {
     uint32_t* klast;
     uint32_t* kparent;  // internal node -> parent
     uint32_t* lparent;  // sorted leaf -> parent
-    uint32_t* flags;    // refit arrival counters
+    uint32_t* flags;    // internal-node depth (edges from the root)
+    uint32_t* lvl;      // refit order: internal nodes by depth (node_count), then REFIT_LEVELS + 1 offsets and
+                        // REFIT_LEVELS counts
+    uint32_t refit_levels;  // host side: depth buckets in use, set by the build
     float4* kbox;       // internal node boxes x 2
     TNode* nodes;
     TNode4* nodes4;     // node_count slots, filled at the wide roots

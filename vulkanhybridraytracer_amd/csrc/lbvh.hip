@@ -297,21 +297,86 @@ __device__ __forceinline__ void child_box(const BuildBuffers& b, uint32_t c, flo
     }
 }
 
-__global__ __launch_bounds__(256) void k_refit(BuildBuffers b) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= b.tri_count) return;
-    uint32_t p = b.lparent[k];
-    for (;;) {
-        uint32_t old = __hip_atomic_fetch_add(&b.flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
+// Box refit, level by level from the deepest internal nodes up (one launch
+// per level): a node's box is the union of its children's, so each level only
+// reads boxes the previous launches wrote. A bottom-up pass with arrival
+// counters would need a device-scope acquire/release per node, which on a
+// GPU with one L2 per XCD writes back and invalidates caches each time.
+// Depths and the per-level node lists depend on the topology only, so the
+// build computes them once and every refit reuses them.
+__global__ __launch_bounds__(256) void k_node_depth(BuildBuffers b, uint32_t nn) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    uint32_t d = 0;
+    for (uint32_t j = i; j != 0u && d < REFIT_LEVELS - 1u; ++d) j = b.kparent[j];
+    b.flags[i] = d;
+}
+
+__global__ __launch_bounds__(256) void k_level_count(BuildBuffers b, uint32_t nn) {
+    __shared__ uint32_t h[REFIT_LEVELS];
+    for (uint32_t k = threadIdx.x; k < REFIT_LEVELS; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) atomicAdd(&h[b.flags[i]], 1u);
+    __syncthreads();
+    uint32_t* counts = b.lvl + nn + REFIT_LEVELS + 1u;
+    for (uint32_t k = threadIdx.x; k < REFIT_LEVELS; k += blockDim.x)
+        if (h[k]) atomicAdd(&counts[k], h[k]);
+}
+
+// offsets (exclusive sums of the counts); the counts become scatter cursors
+__global__ void k_level_offsets(BuildBuffers b, uint32_t nn) {
+    if (threadIdx.x != 0) return;
+    uint32_t* off = b.lvl + nn;
+    uint32_t* counts = off + REFIT_LEVELS + 1u;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < REFIT_LEVELS; ++k) {
+        off[k] = acc;
+        acc += counts[k];
+        counts[k] = off[k];
+    }
+    off[REFIT_LEVELS] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(BuildBuffers b, uint32_t nn) {
+    __shared__ uint32_t h[REFIT_LEVELS], base[REFIT_LEVELS];
+    for (uint32_t k = threadIdx.x; k < REFIT_LEVELS; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d = 0, rank = 0;
+    if (i < nn) {
+        d = b.flags[i];
+        rank = atomicAdd(&h[d], 1u);
+    }
+    __syncthreads();
+    uint32_t* cursor = b.lvl + nn + REFIT_LEVELS + 1u;
+    for (uint32_t k = threadIdx.x; k < REFIT_LEVELS; k += blockDim.x)
+        if (h[k]) base[k] = atomicAdd(&cursor[k], h[k]);
+    __syncthreads();
+    if (i < nn) b.lvl[base[d] + rank] = i;
+}
+
+// deepest depth in use (read back by the build)
+__global__ void k_level_depth(BuildBuffers b, uint32_t nn, uint32_t* out) {
+    if (threadIdx.x != 0) return;
+    const uint32_t* off = b.lvl + nn;
+    uint32_t L = 0;
+    for (uint32_t k = 0; k < REFIT_LEVELS; ++k)
+        if (off[k + 1] > off[k]) L = k + 1u;
+    *out = L;
+}
+
+__global__ __launch_bounds__(256) void k_refit_level(BuildBuffers b, uint32_t nn, uint32_t level) {
+    const uint32_t* off = b.lvl + nn;
+    const uint32_t first = off[level], last = off[level + 1u];
+    for (uint32_t t = first + blockIdx.x * blockDim.x + threadIdx.x; t < last; t += gridDim.x * blockDim.x) {
+        const uint32_t p = b.lvl[t];
         float4 alo, ahi, blo, bhi;
         child_box(b, b.kl[p], alo, ahi);
         child_box(b, b.kr[p], blo, bhi);
         b.kbox[2 * (size_t)p] = make_float4(minsel(alo.x, blo.x), minsel(alo.y, blo.y), minsel(alo.z, blo.z), 0.0f);
         b.kbox[2 * (size_t)p + 1] =
             make_float4(maxsel(ahi.x, bhi.x), maxsel(ahi.y, bhi.y), maxsel(ahi.z, bhi.z), 0.0f);
-        if (p == 0) return;
-        p = b.kparent[p];
     }
 }
 
@@ -502,11 +567,27 @@ hipError_t radix_sort(BuildBuffers& b, hipStream_t st) {
     return hipGetLastError();  // four passes end in keys/ids
 }
 
+// the refit order (depths and per-level node lists) of a new topology; reads
+// back the number of levels, so it synchronises the stream
+hipError_t refit_order(BuildBuffers& b, hipStream_t st) {
+    const uint32_t nn = b.tri_count - 1u;
+    k_node_depth<<<blocks(nn, 256), 256, 0, st>>>(b, nn);
+    k_fill_u32<<<1, 256, 0, st>>>(b.lvl + nn + REFIT_LEVELS + 1u, REFIT_LEVELS, 0u);
+    k_level_count<<<blocks(nn, 256), 256, 0, st>>>(b, nn);
+    k_level_offsets<<<1, 64, 0, st>>>(b, nn);
+    k_level_scatter<<<blocks(nn, 256), 256, 0, st>>>(b, nn);
+    k_level_depth<<<1, 64, 0, st>>>(b, nn, b.depth);
+    hipError_t e = hipMemcpyAsync(&b.refit_levels, b.depth, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+}
+
 hipError_t finish(BuildBuffers& b, hipStream_t st) {
     uint32_t n = b.tri_count;
     if (n >= 2) {
-        k_fill_u32<<<blocks(n - 1, 256), 256, 0, st>>>(b.flags, n - 1, 0u);
-        k_refit<<<blocks(n, 256), 256, 0, st>>>(b);
+        const uint32_t nn = n - 1u;
+        const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+        for (uint32_t L = b.refit_levels; L-- > 0u;) k_refit_level<<<grid, 256, 0, st>>>(b, nn, L);
     }
     uint32_t nn = n >= 2 ? n - 1 : 1;
     k_layout<<<blocks(nn, 256), 256, 0, st>>>(b);
@@ -543,7 +624,11 @@ hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
     k_morton<<<blocks(n, 256), 256, 0, st>>>(b);
     hipError_t e = radix_sort(b, st);
     if (e != hipSuccess) return e;
-    if (n >= 2) k_karras<<<blocks(n - 1, 256), 256, 0, st>>>(b);
+    if (n >= 2) {
+        k_karras<<<blocks(n - 1, 256), 256, 0, st>>>(b);
+        e = refit_order(b, st);
+        if (e != hipSuccess) return e;
+    }
     return finish(b, st);
 }
 
