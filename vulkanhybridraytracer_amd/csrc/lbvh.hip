@@ -594,13 +594,15 @@ hipError_t finish(BuildBuffers& b, hipStream_t st) {
     // wide levels: roots alternate between the radix sort's spare key and id
     // arrays (n entries each; a level has fewer than n roots), counts in the
     // histogram buffer. A wide level descends at least one binary level, so
-    // MAX_DEPTH levels reach every leaf of a tree that passes the depth check.
+    // as many levels as the binary tree has internal-node depths reach every
+    // leaf (one for a single triangle).
     {
         uint32_t* roots[2] = {b.keys_tmp, b.ids_tmp};
         uint32_t* counts = b.hist;
         const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+        const uint32_t levels = std::max(b.refit_levels, 1u);
         k_collapse_begin<<<1, 64, 0, st>>>(roots[0], counts);
-        for (uint32_t level = 0; level < MAX_DEPTH; ++level) {
+        for (uint32_t level = 0; level < levels; ++level) {
             const uint32_t a = level & 1u;
             k_collapse_level<<<grid, 256, 0, st>>>(b, roots[a], counts + a, roots[a ^ 1u], counts + (a ^ 1u));
             k_collapse_consumed<<<1, 64, 0, st>>>(counts + a);
