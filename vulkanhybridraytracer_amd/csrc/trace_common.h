@@ -662,6 +662,25 @@ __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, 
 // (atomicOr, so the result does not depend on the order).
 enum : int { ITEM_SKIP = 0, ITEM_LIT = 1, ITEM_TRACE = 2 };
 
+// Bit `item` of `bits` for every calling lane with `set`, one global atomicOr
+// per distinct word: lanes that finish together mostly hold consecutive items.
+// The word's bits are gathered with LDS ORs in the wave's `slot` (a shuffle
+// reduction would read the stale registers of inactive lanes).
+__device__ __forceinline__ void or_item_bits(uint32_t* bits, uint32_t item, bool set, uint32_t* slot) {
+    unsigned long long pend = __ballot(set);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = item >> 5;
+    while (pend) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)pend) - 1u;
+        const uint32_t word = (uint32_t)__shfl((int)w, (int)leader, 64);
+        const bool mine = set && w == word;
+        if (lane == leader) *slot = 0u;
+        if (mine) atomicOr(slot, 1u << (item & 31u));
+        if (lane == leader) atomicOr(bits + word, *slot);
+        pend &= ~__ballot(mine);
+    }
+}
+
 template <class Src>
 __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* next_item, uint32_t* bits,
                                           uint32_t* lds, Cnt& c) {
@@ -680,13 +699,14 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
     const float tmin = 0.001f;
     int sp = 0;
     uint32_t spill[MAX_STACK4 - STACK_LDS];
+    __shared__ uint32_t or_slot[BLOCK / 64];
+    uint32_t* slot = or_slot + (threadIdx.x >> 6);
     for (;;) {
         if (cur == DONE && leaf == 0u) {
-            if (tracing) {  // the ray of `item` ended
-                if (hit) ++c.occluded;
-                else atomicOr(bits + (item >> 5), 1u << (item & 31u));
-                tracing = false;
-            }
+            // the ray of `item` ended
+            if (tracing && hit) ++c.occluded;
+            or_item_bits(bits, item, tracing && !hit, slot);
+            tracing = false;
             bool exhausted = false;
             for (;;) {
                 uint32_t k = 0;
@@ -711,11 +731,8 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                     continue;
                 }
                 const int kind = src.setup(item, r, tmax, nodes4, tris, node_count, c);
-                if (kind == ITEM_SKIP) continue;
-                if (kind == ITEM_LIT) {
-                    atomicOr(bits + (item >> 5), 1u << (item & 31u));
-                    continue;
-                }
+                or_item_bits(bits, item, kind == ITEM_LIT, slot);
+                if (kind != ITEM_TRACE) continue;
                 tracing = true;
                 hit = false;
                 sp = 0;
