@@ -83,6 +83,9 @@ __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur
     const gf4 np = gptr(nodes4 + cur);
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     const uint4 cd = ((gu4)np)[6];
+    // The stack top, read while the node loads are in flight: a node with no
+    // hit child then pops it without a second round trip.
+    const uint32_t top = (sp > 0 && sp <= (int)STACK_LDS) ? lds[(sp - 1) * BLOCK] : DONE;
     const float inf = __uint_as_float(0x7f800000u);
     float t0, t1, t2, t3;
     const bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, t0) && cd.x != EMPTY4;
@@ -100,7 +103,7 @@ __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur
     if (nh == 0u) {
         if (sp == 0) return DONE;
         --sp;
-        return sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+        return sp < (int)STACK_LDS ? top : spill[sp - STACK_LDS];
     }
     auto push = [&](uint32_t code) {
         if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
