@@ -24,6 +24,23 @@ using namespace hrd;
 using namespace tc;
 
 namespace {
+// Pixel (x, y) of pixel q of shadow tile t (tiles of tile_w x 64 / tile_w
+// pixels). Tiles run row by row down vertical bands of TILE_BAND tile
+// columns, one band after another, so the tiles in flight at once cover a
+// compact patch of the screen rather than a thin strip. (x, y) may lie
+// beyond the frame's right or bottom edge.
+constexpr uint32_t TILE_BAND = 8u;
+__device__ __forceinline__ void tile_pixel(const ShadowJob& J, uint32_t t, uint32_t q, uint32_t& x, uint32_t& y) {
+    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
+    const uint32_t tiles_y = (J.F.height + th - 1u) / th;
+    const uint32_t per_band = TILE_BAND * tiles_y;
+    const uint32_t band = t / per_band, k = t - band * per_band;
+    const uint32_t bw = min(TILE_BAND, J.tiles_x - band * TILE_BAND);
+    const uint32_t ty = k / bw, tx = band * TILE_BAND + (k - ty * bw);
+    x = tx * tw + (q % tw);
+    y = ty * th + (q / tw);
+}
+
 // World-space direction of the primary ray through the centre of pixel (x, y)
 // (raygen.rgen:29-40); x and y may lie outside the image.
 __device__ __forceinline__ v3 primary_dir(const FrameArgs& F, uint32_t x, uint32_t y) {
@@ -271,8 +288,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     uint32_t t = lg / per_tile, rem = lg - t * per_tile;
                     li = rem >> 6;
                     uint32_t q = rem & 63u;
-                    uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-                    uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+                    uint32_t x, y;
+                    tile_pixel(J, t, q, x, y);
                     if (x >= F.width || y >= F.height) continue;
                     size_t p = (size_t)y * F.width + x;
                     out = J.shadow + li * ((size_t)F.width * F.height) + p;
@@ -528,8 +545,8 @@ struct HybridItems {
         const uint32_t per_tile = F.nl * 64u;
         const uint32_t t = g / per_tile, rem = g - t * per_tile;
         const uint32_t li = rem >> 6, q = rem & 63u;
-        const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-        const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+        uint32_t x, y;
+        tile_pixel(J, t, q, x, y);
         if (x >= F.width || y >= F.height) return ITEM_SKIP;
         const bool hard = MODE == 1 && hard_light(F, li);
         if (hard && s > 0u) return ITEM_SKIP;  // a hard light takes one probe
@@ -596,8 +613,8 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
-    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    uint32_t x, y;
+    tile_pixel(J, t, q, x, y);
     if (x >= F.width || y >= F.height) return;
     const uint32_t n = window_bits(bits, (uint64_t)G << lg2s, lg2s);
     J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
@@ -618,8 +635,8 @@ __global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restri
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
-    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    uint32_t x, y;
+    tile_pixel(J, t, q, x, y);
     if (x >= F.width || y >= F.height) return;
     const uint32_t w = (bits[G >> 3] >> ((G & 7u) * 4u)) & 15u;  // the 4 probe bits of the group
     float* out = J.shadow + li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
@@ -649,8 +666,8 @@ __global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __r
     const uint32_t per_tile = F.nl * 64u;
     const uint32_t t = g / per_tile, rem = g - t * per_tile;
     const uint32_t li = rem >> 6, q = rem & 63u;
-    const uint32_t tw = J.tile_w, th = 64u / J.tile_w;
-    const uint32_t x = (t % J.tiles_x) * tw + (q % tw), y = (t / J.tiles_x) * th + (q / tw);
+    uint32_t x, y;
+    tile_pixel(J, t, q, x, y);
     const uint32_t n = window_bits(bits, (uint64_t)slot << lg2s, lg2s);
     J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
 }
