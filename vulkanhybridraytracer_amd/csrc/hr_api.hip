@@ -412,7 +412,7 @@ hr_status hr_context_create(int device, hr_context** out) {
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[1]);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&ctx->d_next_group, 64);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_next_group, BAND_COUNTER_BYTES);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)

@@ -16,7 +16,11 @@ constexpr uint32_t STACK_LDS = 16;       // entries per lane kept in LDS
 constexpr uint32_t STACK_SPILL = MAX_DEPTH - STACK_LDS;  // entries per lane in private memory
 constexpr uint32_t LEAF_MAX_LIMIT = 16;  // leaf code holds count - 1 in 4 bits
 constexpr uint32_t MAX_TRIS = 1u << 27;  // leaf code holds the first triangle in 27 bits
-constexpr uint32_t SHADOW_QUEUES = 8;    // shadow work bands, one per XCD (counters in a 64-byte buffer)
+constexpr uint32_t SHADOW_QUEUES = 8;    // shadow work bands, one per XCD
+// Band b's counter is word b * BAND_STRIDE of its buffer: 256 B apart, since
+// device-scope atomics are performed at memory and serialise per cache line.
+constexpr uint32_t BAND_STRIDE = 64;
+constexpr size_t BAND_COUNTER_BYTES = (size_t)SHADOW_QUEUES * BAND_STRIDE * 4u;
 
 __host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) { return LEAF_BIT | ((count - 1u) << 27) | first; }
 
@@ -152,9 +156,9 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive);
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                               bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
                               uint32_t* scratch, uint32_t resident_blocks);
-// jobs_dev: njobs ShadowJob in device memory; next_group: SHADOW_QUEUES device
-// uint32 band counters (reset by the launch); resident_blocks: the persistent
-// grid size
+// jobs_dev: njobs ShadowJob in device memory; next_group: BAND_COUNTER_BYTES
+// of device memory holding the SHADOW_QUEUES band counters, BAND_STRIDE words
+// apart (reset by the launch); resident_blocks: the persistent grid size
 hipError_t launch_shadows(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, unsigned long long* counters,
                           bool count, hipStream_t st, uint32_t* next_group, uint32_t resident_blocks);
 // Workspace of the wavefront reflection pipeline for one slab of pixels

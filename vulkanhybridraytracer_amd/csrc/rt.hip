@@ -307,7 +307,7 @@ __global__ void k_rt_reset(RtWork w, uint32_t depth, uint32_t* next_item) {
         w.cnt[2] = 0u;
         w.cnt[(depth + 1u) & 1u] = 0u;
     }
-    if (threadIdx.x < SHADOW_QUEUES) next_item[threadIdx.x] = 0u;
+    if (threadIdx.x < SHADOW_QUEUES) next_item[threadIdx.x * BAND_STRIDE] = 0u;
 }
 
 // wave-aggregated append: slot of this lane among the wave's takers

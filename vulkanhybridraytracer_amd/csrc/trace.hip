@@ -266,7 +266,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     uint32_t rank = (uint32_t)__popcll(need & lt_mask);
                     uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
                     uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(next_group + band, (uint32_t)__popcll(need));
+                    if (lane == leader) base = atomicAdd(next_group + band * BAND_STRIDE, (uint32_t)__popcll(need));
                     base = (uint32_t)__shfl((int)base, (int)leader, 64);
                     uint32_t g = (uint32_t)(((uint64_t)ngroups * band) / SHADOW_QUEUES) + base + rank;
                     if (g >= band_end) {  // the band is empty: move to the next one
@@ -479,8 +479,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DevScene S, const hr_ray* rays,
     flush_counters(c, counters, COUNT);
 }
 
+// zero the band counters (word b * BAND_STRIDE for band b)
 __global__ void k_reset_u32(uint32_t* p) {
-    if (threadIdx.x < SHADOW_QUEUES) p[threadIdx.x] = 0u;
+    if (threadIdx.x < SHADOW_QUEUES) p[threadIdx.x * BAND_STRIDE] = 0u;
 }
 
 __global__ void k_fill_bits(uint32_t* p, uint32_t n) {

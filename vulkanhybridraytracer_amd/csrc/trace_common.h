@@ -737,7 +737,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                 uint32_t rank = (uint32_t)__popcll(need & lt_mask);
                 uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(next_item + band, (uint32_t)__popcll(need));
+                if (lane == leader) base = atomicAdd(next_item + band * BAND_STRIDE, (uint32_t)__popcll(need));
                 base = (uint32_t)__shfl((int)base, (int)leader, 64);
                 item = (uint32_t)(((uint64_t)nitems * band) / SHADOW_QUEUES) + base + rank;
                 if (item >= band_end) {
