@@ -261,6 +261,7 @@ size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char*
     size_t o_p0 = carve(slots * 48), o_p1 = carve(slots * 48), o_hits = carve(slots * 16);
     size_t o_sr = carve(slots * f.nl * 32), o_vis = carve(slots * 4), o_pend = carve(slots * 4);
     size_t o_col = carve(slots * 16), o_pix = carve(slab * 32), o_cnt = carve(64);
+    size_t o_take = carve(2 * BAND_COUNTER_BYTES);
     if (b) {
         w.paths0 = reinterpret_cast<float4*>(b + o_p0);
         w.paths1 = reinterpret_cast<float4*>(b + o_p1);
@@ -271,11 +272,14 @@ size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char*
         w.pathcol = reinterpret_cast<float4*>(b + o_col);
         w.pix = reinterpret_cast<float4*>(b + o_pix);
         w.counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+        w.takes = reinterpret_cast<uint32_t*>(b + o_take);
         w.p0 = 0;
         w.p1 = 0;
         w.spp_max = spp_max;
         w.slab_pixels = (uint32_t)slab;
-        w.blocks = ctx->resident_blocks;
+        // the trace stages' persistent grid, at most one lane per queue slot:
+        // every wave of it takes (and at the end fails) at least once
+        w.blocks = (uint32_t)std::min<size_t>(ctx->resident_blocks, (slots + 255) / 256);
     }
     return slots >= (size_t(1) << 28) ? 0 : off;
 }

@@ -412,22 +412,40 @@ __device__ __forceinline__ void flush_counters(const Cnt& c, unsigned long long*
     }
 }
 
-// Wave-aggregated take of one index per calling lane from *next: one atomic
-// for all lanes of the wave that reach this call together.
-__device__ __forceinline__ uint32_t wave_take(uint32_t* next) {
+// Wave-aggregated take of one index of [0, n) per calling lane. The range is
+// cut into SHADOW_QUEUES bands with their counters BAND_STRIDE words apart in
+// `next` (one atomic per band for all lanes of the wave that reach this call
+// together; counters on separate lines do not serialise one another). A wave
+// starts at its home band and moves on when a band runs dry; `emptied` (per
+// lane, reduced over the wave by ballot so all lanes agree) counts the bands
+// found empty. Returns n once every band is empty.
+template <uint32_t NB = SHADOW_QUEUES>
+__device__ __forceinline__ uint32_t band_take(uint32_t* next, uint32_t n, uint32_t home, uint32_t& emptied) {
     const uint32_t lane = threadIdx.x & 63u;
-    unsigned long long need = __ballot(1);
-    uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-    uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(next, (uint32_t)__popcll(need));
-    base = (uint32_t)__shfl((int)base, (int)leader, 64);
-    return base + rank;
+    for (;;) {
+        uint32_t k = 0;
+        for (uint32_t b = 1; b <= NB; ++b)
+            if (__ballot(emptied >= b)) k = b;
+        emptied = k;
+        if (k >= NB) return n;
+        const uint32_t band = (home + k) & (NB - 1u);
+        const uint32_t band_end = (uint32_t)(((uint64_t)n * (band + 1u)) / NB);
+        const unsigned long long need = __ballot(1);
+        const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+        const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(next + band * BAND_STRIDE, (uint32_t)__popcll(need));
+        base = (uint32_t)__shfl((int)base, (int)leader, 64);
+        const uint32_t idx = (uint32_t)(((uint64_t)n * band) / NB) + base + rank;
+        if (idx < band_end) return idx;
+        emptied = k + 1u;
+    }
 }
 
 // Persistent consumer of a ray queue of length n. Each lane traces ray after
-// ray; when its ray finishes it takes the next index from *next, so lanes do
-// not wait for the slowest ray of the wave. Per ray the walk is the shared
+// ray; when its ray finishes it takes the next index (band_take over the
+// SHADOW_QUEUES band counters at `next`), so lanes do not wait for the
+// slowest ray of the wave. Per ray the walk is the shared
 // near-first stack traversal (same order and counts as traverse()).
 //   get(i, o, d, tmin, tmax) -> false for an empty queue entry (skipped)
 //   put(i, hit, HitD)         result of entry i
@@ -448,6 +466,8 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
     // do not depend on the order leaves are tested in
     constexpr bool SPEC = !COUNT;
     uint32_t leaf = 0u;  // postponed leaf code (0: none)
+    const uint32_t home = blockIdx.x & (SHADOW_QUEUES - 1u);
+    uint32_t emptied = 0;
     for (;;) {
         if (cur == DONE && leaf == 0u) {
             if (have) {
@@ -461,7 +481,7 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
             }
             bool got = false;
             for (;;) {
-                idx = wave_take(next);
+                idx = band_take(next, n, home, emptied);
                 if (idx >= n) break;
                 v3 o, d;
                 float tmax;

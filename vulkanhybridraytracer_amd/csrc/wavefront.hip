@@ -523,9 +523,12 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_resolve(FrameArgs F, GBufView G
     refl[p] = make_float4(rfac.x, rfac.y, rfac.z, 1.0f);
 }
 
-__global__ void k_wf_reset(uint32_t* counts, uint32_t mask) {
+// counts[i] for the bits i of mask; with `takes`, also the band take counters
+// of both trace stages
+__global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* takes) {
     uint32_t i = threadIdx.x;
     if (i < 16 && ((mask >> i) & 1u)) counts[i] = 0u;
+    if (takes && i < 2 * SHADOW_QUEUES) takes[i * BAND_STRIDE] = 0u;
 }
 
 constexpr uint32_t STAGE_BLOCKS = 2048;  // 8 workgroups per CU, each striding over the queue
@@ -544,7 +547,7 @@ hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, cons
     for (uint32_t p0 = 0; p0 < npix; p0 += slab) {
         w.p0 = p0;
         w.p1 = min(npix, p0 + slab);
-        k_wf_reset<<<1, 64, 0, st>>>(w.counts, 0xFu);
+        k_wf_reset<<<1, 64, 0, st>>>(w.counts, 0xFu, nullptr);
         uint32_t nb = (w.p1 - w.p0 + WF_BLOCK - 1) / WF_BLOCK;
         k_wf_direct<<<nb, WF_BLOCK, 0, st>>>(s, f, gb, shadow, lit, refl, w, counters);
         for (uint32_t d = 0; d < f.max_depth; ++d) {
@@ -552,13 +555,14 @@ hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, cons
             float4* nxt = (d & 1) ? w.paths0 : w.paths1;
             uint32_t* ccur = &w.counts[d & 1];
             uint32_t* cnxt = &w.counts[(d + 1) & 1];
-            // counts[4] / counts[5]: take counters of the two trace stages
-            k_wf_reset<<<1, 64, 0, st>>>(w.counts, (1u << ((d + 1) & 1)) | 0x3Cu);
-            if (count) k_wf_trace<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, &w.counts[4], counters);
-            else k_wf_trace<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, &w.counts[4], counters);
+            uint32_t* take_trace = w.takes;
+            uint32_t* take_shadow = w.takes + SHADOW_QUEUES * BAND_STRIDE;
+            k_wf_reset<<<1, 64, 0, st>>>(w.counts, (1u << ((d + 1) & 1)) | 0xCu, w.takes);
+            if (count) k_wf_trace<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, take_trace, counters);
+            else k_wf_trace<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, cur, ccur, w.hits, take_trace, counters);
             k_wf_shade<<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, f, w, cur, nxt, ccur, cnxt, d);
-            if (count) k_wf_shadow<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, &w.counts[5], counters);
-            else k_wf_shadow<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, &w.counts[5], counters);
+            if (count) k_wf_shadow<true><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, take_shadow, counters);
+            else k_wf_shadow<false><<<w.blocks, WF_BLOCK, 0, st>>>(s, w, take_shadow, counters);
             k_wf_finish<<<STAGE_BLOCKS, WF_BLOCK, 0, st>>>(s, f, w, cur, d);
         }
         k_wf_resolve<<<nb, WF_BLOCK, 0, st>>>(f, gb, w, lit, refl);
