@@ -194,7 +194,7 @@ GBufView gview(const hr_gbuffer* gb) {
 // stats the call synchronises and reports counts and milliseconds.
 hr_status frame_begin(hr_context* ctx, bool timing) {
     HR_HIP(ctx, hipSetDevice(ctx->device));
-    HR_HIP(ctx, launch_zero_u64(ctx->d_counters, C_COUNT, ctx->stream));
+    HR_HIP(ctx, launch_zero_u64(ctx->d_counters, C_COUNT * C_STRIDE, ctx->stream));
     if (timing) HR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     return HR_OK;
 }
@@ -202,8 +202,9 @@ hr_status frame_end(hr_context* ctx, hr_stats* stats, float* ms_slot) {
     if (!stats) return HR_OK;
     HR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     HR_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
-    unsigned long long c[C_COUNT];
-    HR_HIP(ctx, hipMemcpy(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+    unsigned long long cs[C_COUNT * C_STRIDE], c[C_COUNT];
+    HR_HIP(ctx, hipMemcpy(cs, ctx->d_counters, sizeof(cs), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < C_COUNT; ++k) c[k] = cs[k * C_STRIDE];
     float ms = 0.0f;
     HR_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     float keep[4] = {stats->ms_gbuffer, stats->ms_shadow, stats->ms_shade, stats->ms_build};
@@ -415,7 +416,7 @@ hr_status hr_context_create(int device, hr_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[1]);
-    if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * C_STRIDE * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_next_group, BAND_COUNTER_BYTES);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -1071,7 +1072,7 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     hr_graph* g = new hr_graph();
     g->ctx = ctx;
     hipError_t e = hipMalloc(&g->d_jobs, n * sizeof(ShadowJob));
-    if (e == hipSuccess) e = hipMalloc(&g->d_next, 64);
+    if (e == hipSuccess) e = hipMalloc(&g->d_next, BAND_COUNTER_BYTES);
     if (e == hipSuccess) e = hipMalloc(&g->work, bytes);
     if (e == hipSuccess) e = hipMemcpy(g->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -1085,7 +1086,7 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
         return hip_fail(ctx, e, "hipStreamBeginCapture");
     }
     hr_status st = HR_OK;
-    hipError_t le = launch_zero_u64(ctx->d_counters, C_COUNT, ctx->stream);
+    hipError_t le = launch_zero_u64(ctx->d_counters, C_COUNT * C_STRIDE, ctx->stream);
     if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
     if (st == HR_OK)
         st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work));

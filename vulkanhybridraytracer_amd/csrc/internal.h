@@ -131,6 +131,10 @@ enum : int {
     C_OVERFLOW = 7,
     C_COUNT = 8
 };
+// Counter k is word k * C_STRIDE of the counter buffer (256 B apart): every
+// wave of every kernel adds its totals at its end, and device-scope atomics
+// to one cache line serialise.
+constexpr uint32_t C_STRIDE = 32;
 
 // ---- launchers (trace.hip) ---------------------------------------------------
 hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, unsigned long long* counters,

@@ -408,7 +408,7 @@ __device__ __forceinline__ void flush_counters(const Cnt& c, unsigned long long*
     for (int k = 0; k < 8; ++k) {
         if (!count && (k == 5 || k == 6)) continue;
         uint32_t s = wave_sum(v[k]);
-        if ((threadIdx.x & 63u) == 0 && s) atomicAdd(&counters[slot[k]], (unsigned long long)s);
+        if ((threadIdx.x & 63u) == 0 && s) atomicAdd(&counters[slot[k] * C_STRIDE], (unsigned long long)s);
     }
 }
 
