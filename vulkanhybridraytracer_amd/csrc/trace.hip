@@ -589,7 +589,7 @@ struct HybridItems {
 };
 
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_shadows_spl(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
     uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
     unsigned long long* counters) {
