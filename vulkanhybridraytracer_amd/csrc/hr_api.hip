@@ -261,7 +261,7 @@ size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char*
     };
     size_t o_p0 = carve(slots * 48), o_p1 = carve(slots * 48), o_hits = carve(slots * 16);
     size_t o_sr = carve(slots * f.nl * 32), o_vis = carve(slots * 4), o_pend = carve(slots * 4);
-    size_t o_col = carve(slots * 16), o_pix = carve(slab * 32), o_cnt = carve(64);
+    size_t o_col = carve(slots * 16), o_pix = carve(slab * 32), o_cnt = carve(4 * BAND_STRIDE * 4);
     size_t o_take = carve(2 * BAND_COUNTER_BYTES);
     if (b) {
         w.paths0 = reinterpret_cast<float4*>(b + o_p0);

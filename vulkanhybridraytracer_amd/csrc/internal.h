@@ -177,7 +177,7 @@ struct WaveArgs {
     uint32_t* pending;
     float4* pathcol;
     float4* pix;
-    uint32_t* counts;  // [0], [1] path queues, [2] shadow rays, [3] pending
+    uint32_t* counts;  // queue counters, word i * BAND_STRIDE: 0, 1 path queues, 2 shadow rays, 3 pending
     uint32_t* takes;   // band take counters of the two trace stages: 2 x BAND_COUNTER_BYTES
     uint32_t p0, p1, spp_max, slab_pixels;
     uint32_t blocks;   // persistent grid of the trace stages

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
         }
     }
     // reserve queue slots for this wave's paths, then write them
-    uint32_t base = wave_push(&W.counts[0], npush);
+    uint32_t base = wave_push(&W.counts[0 * BAND_STRIDE], npush);
     if (npush) {
         bool glossy = mode == 3 && roughness > 0.0f;
         uint32_t k = 0;
@@ -426,12 +426,12 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_shade(DevScene S, FrameArgs F, 
             }
         }
         // pushes (whole wave participates)
-        uint32_t pbase = wave_push(&W.counts[3], pending ? 1u : 0u);
+        uint32_t pbase = wave_push(&W.counts[3 * BAND_STRIDE], pending ? 1u : 0u);
         if (pending) {
             W.pending[pbase] = i;
             W.vis[i] = 0u;
         }
-        uint32_t sbase = wave_push(&W.counts[2], nshadow);
+        uint32_t sbase = wave_push(&W.counts[2 * BAND_STRIDE], nshadow);
         if (nshadow) {
             uint32_t k = 0;
             for (uint32_t l = 0; l < F.nl; ++l) {
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_shadow(DevScene S, WaveArgs W, 
     __shared__ uint32_t stk[STACK_LDS * WF_BLOCK];
     uint32_t* lds = stk + threadIdx.x;
     Cnt c = {};
-    uint32_t n = W.counts[2];
+    uint32_t n = W.counts[2 * BAND_STRIDE];
     auto get = [&](uint32_t i, v3& o, v3& d, float& tmin, float& tmax) {
         float4 a = W.srays[2 * (size_t)i], b = W.srays[2 * (size_t)i + 1];
         o = ld3(a);
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_shadow(DevScene S, WaveArgs W, 
 
 __global__ __launch_bounds__(WF_BLOCK) void k_wf_finish(DevScene S, FrameArgs F, WaveArgs W, const float4* paths,
                                                         uint32_t depth) {
-    uint32_t n = W.counts[3];
+    uint32_t n = W.counts[3 * BAND_STRIDE];
     uint32_t stride = gridDim.x * WF_BLOCK;
     for (uint32_t k = blockIdx.x * WF_BLOCK + threadIdx.x; k < n; k += stride) {
         uint32_t i = W.pending[k];
@@ -523,11 +523,11 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_resolve(FrameArgs F, GBufView G
     refl[p] = make_float4(rfac.x, rfac.y, rfac.z, 1.0f);
 }
 
-// counts[i] for the bits i of mask; with `takes`, also the band take counters
-// of both trace stages
+// queue counter i (word i * BAND_STRIDE) for the bits i of mask; with
+// `takes`, also the band take counters of both trace stages
 __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* takes) {
     uint32_t i = threadIdx.x;
-    if (i < 16 && ((mask >> i) & 1u)) counts[i] = 0u;
+    if (i < 4 && ((mask >> i) & 1u)) counts[i * BAND_STRIDE] = 0u;
     if (takes && i < 2 * SHADOW_QUEUES) takes[i * BAND_STRIDE] = 0u;
 }
 
@@ -553,8 +553,8 @@ hipError_t launch_shade(const DevScene& s, const FrameArgs& f, GBufView gb, cons
         for (uint32_t d = 0; d < f.max_depth; ++d) {
             float4* cur = (d & 1) ? w.paths1 : w.paths0;
             float4* nxt = (d & 1) ? w.paths0 : w.paths1;
-            uint32_t* ccur = &w.counts[d & 1];
-            uint32_t* cnxt = &w.counts[(d + 1) & 1];
+            uint32_t* ccur = &w.counts[(d & 1) * BAND_STRIDE];
+            uint32_t* cnxt = &w.counts[((d + 1) & 1) * BAND_STRIDE];
             uint32_t* take_trace = w.takes;
             uint32_t* take_shadow = w.takes + SHADOW_QUEUES * BAND_STRIDE;
             k_wf_reset<<<1, 64, 0, st>>>(w.counts, (1u << ((d + 1) & 1)) | 0xCu, w.takes);
