@@ -108,17 +108,6 @@ __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur
         if (sp >= (int)STACK_LDS) return spill[sp - STACK_LDS];
         return TOP_AHEAD ? top : lds[sp * BLOCK];
     }
-    if (sp + 3 <= (int)STACK_LDS) {
-        // Room for three in LDS: no branches. Child j (1..3) goes below the
-        // new top when it is pushed (j < nh: slot nh - 1 - j) and otherwise
-        // into a free slot above it (slot j - 1); the slots are distinct.
-        uint32_t* base = lds + sp * BLOCK;
-        base[(1u < nh ? nh - 2u : 0u) * BLOCK] = c1;
-        base[(2u < nh ? nh - 3u : 1u) * BLOCK] = c2;
-        base[(3u < nh ? nh - 4u : 2u) * BLOCK] = c3;
-        sp += (int)nh - 1;
-        return c0;
-    }
     auto push = [&](uint32_t code) {
         if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
         else if (sp < (int)CAP) spill[sp - STACK_LDS] = code;
