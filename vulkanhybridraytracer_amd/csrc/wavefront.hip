@@ -456,7 +456,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_shade(DevScene S, FrameArgs F, 
 
 // any hit of every hit shadow ray; unoccluded rays set their light's bit
 template <bool COUNT>
-__global__ __launch_bounds__(WF_BLOCK) void k_wf_shadow(DevScene S, WaveArgs W, uint32_t* next,
+__global__ __launch_bounds__(WF_BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_wf_shadow(DevScene S, WaveArgs W, uint32_t* next,
                                                         unsigned long long* counters) {
     __shared__ uint32_t stk[STACK_LDS * WF_BLOCK];
     uint32_t* lds = stk + threadIdx.x;
