@@ -369,7 +369,9 @@ void hr_graph_destroy(hr_graph* graph);
  *  blocks_per_cu   persistent trace grid, 256-thread workgroups per CU (1..16)
  *  tile_w          shadow work tile width in pixels (1..64, power of two;
  *                  height 64 / tile_w): one wave's 64 groups
- *  wave_budget_mb  reflection-queue workspace per pixel slab, MiB (>= 64) */
+ *  wave_budget_mb  reflection-queue workspace per pixel slab, MiB (>= 64;
+ *                  default 40960: a 1080p frame at 64 samples per pixel is one
+ *                  slab) */
 typedef struct hr_tuning {
     uint32_t blocks_per_cu;
     uint32_t tile_w;

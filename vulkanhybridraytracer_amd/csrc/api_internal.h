@@ -17,7 +17,7 @@ struct hr_context {
     uint32_t* d_next_group = nullptr;  // work counter of the persistent shadow kernel
     uint32_t cu_count = 256;
     uint32_t resident_blocks = 2048;    // persistent grid: cu_count x tuning.blocks_per_cu
-    hr_tuning tuning{8u, 8u, 16384u, 0u};  // 16 GiB reflection workspace at most (1080p at 64 samples per pixel: 3 slabs)
+    hr_tuning tuning{8u, 8u, 40960u, 0u};  // 40 GiB reflection workspace at most (1080p at 64 samples per pixel: one slab)
     void* work = nullptr;  // grow-only workspace of the reflection pipeline
     size_t work_bytes = 0;
     hr::ShadowJob* d_jobs = nullptr;  // grow-only device copy of the shadow jobs of a call

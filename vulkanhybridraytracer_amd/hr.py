@@ -175,7 +175,7 @@ class Context:
         self.check(self.L.hr_post_gamma(self.h, width, height, src.ptr, None if out is None else out.ptr,
                                         None if rgba8 is None else rgba8.ptr))
 
-    def set_tuning(self, blocks_per_cu=8, tile_w=8, wave_budget_mb=16384):
+    def set_tuning(self, blocks_per_cu=8, tile_w=8, wave_budget_mb=40960):
         t = abi.hr_tuning(blocks_per_cu, tile_w, wave_budget_mb, 0)
         self.check(self.L.hr_context_set_tuning(self.h, C.byref(t)))
 
