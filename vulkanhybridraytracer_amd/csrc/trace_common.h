@@ -137,7 +137,7 @@ __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tma
         float bu = 0.0f, bv = 0.0f;
         for (;;) {
             if (cur < LEAF_BIT) {
-                cur = wide_step<MAX_STACK4>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
+                cur = wide_step<MAX_STACK4, ANY>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
                 continue;
             }
             if (cur == DONE) break;
