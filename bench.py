@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--workload", default="hybrid_1080p_1m")
     ap.add_argument("--cpu-rows-step", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--leaf-max", type=int, default=4, help="largest subtree stored as one BVH leaf")
+    ap.add_argument("--leaf-max", type=int, default=2, help="largest subtree stored as one BVH leaf")
     ap.add_argument("--wave-budget-mb", type=int, default=0,
                     help="reflection workspace budget in MiB (0: the library default)")
     ap.add_argument("--batch-count", type=int, default=32)
@@ -327,7 +327,7 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         gb = rd.gb.download()
         sh = rd.download_shadow()
-        os_ = pyoracle.OracleScene(sc, threads=threads)
+        os_ = pyoracle.OracleScene(sc, leaf_max=args.leaf_max, threads=threads)
         rows = (0, args.cpu_rows_step)
         t1 = time.perf_counter()
         _, c1 = os_.trace_shadows(gb, rows=rows)

@@ -240,7 +240,7 @@ typedef struct hr_stats {
 #define HR_FLAG_ADAPTIVE 4u
 
 typedef struct hr_build_params {
-    uint32_t leaf_max;   /* largest Karras subtree stored as one leaf (1..8, default 4) */
+    uint32_t leaf_max;   /* largest Karras subtree stored as one leaf (1..8, default 2) */
     uint32_t _reserved[3];
 } hr_build_params;
 

@@ -99,7 +99,7 @@ def _p(a):
 class OracleScene:
     """CPU LBVH scene built from a scenes.Scene (same inputs as hr_scene_build)."""
 
-    def __init__(self, scene, leaf_max=4, threads=0):
+    def __init__(self, scene, leaf_max=2, threads=0):
         self.scene = scene
         self.threads = threads
         self.packed = abi.PackedScene(scene)

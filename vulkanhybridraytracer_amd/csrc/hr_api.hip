@@ -16,7 +16,7 @@ using namespace hr;
 
 struct hr_scene {
     hr_context* ctx = nullptr;
-    uint32_t mesh_count = 0, prim_count = 0, ntri = 0, nmat = 0, leaf_max = 4, depth = 0, node_count = 0;
+    uint32_t mesh_count = 0, prim_count = 0, ntri = 0, nmat = 0, leaf_max = 2, depth = 0, node_count = 0;
     float ms_build = 0.0f;
     std::vector<PrimInfo> h_prims;
     std::vector<float> h_models;
@@ -510,7 +510,7 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     if ((mesh_count && !meshes) || (prim_count && !prims))
         return fail(ctx, HR_E_INVALID, "meshes/primitives pointer is NULL");
     if (!materials || material_count == 0) return fail(ctx, HR_E_INVALID, "at least one material is required");
-    uint32_t leaf_max = params && params->leaf_max ? params->leaf_max : 4u;
+    uint32_t leaf_max = params && params->leaf_max ? params->leaf_max : 2u;
     if (leaf_max > LEAF_MAX_LIMIT) return fail(ctx, HR_E_INVALID, "leaf_max %u exceeds %u", leaf_max, LEAF_MAX_LIMIT);
     for (uint32_t i = 0; i < material_count; ++i) HR_TRY(check_material(ctx, materials[i], i, material_count));
 

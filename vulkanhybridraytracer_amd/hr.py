@@ -226,7 +226,7 @@ class DeviceBuffer:
 class HybridScene:
     """A scene on the device: world-space triangles and the GPU LBVH."""
 
-    def __init__(self, ctx, scene, leaf_max=4):
+    def __init__(self, ctx, scene, leaf_max=2):
         self.ctx = ctx
         self.scene = scene
         self.packed = abi.PackedScene(scene)
