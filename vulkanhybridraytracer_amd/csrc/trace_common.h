@@ -56,7 +56,7 @@ constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (few
 // consume_rays (queues of closest-hit and hit-shadow rays) and item_loop
 // (sample-parallel shadow items) have their own.
 constexpr int REFILL_MIN_LANES = 16;
-constexpr int REFILL_MIN_ITEMS = 32;
+constexpr int REFILL_MIN_ITEMS = 40;
 
 // Node and triangle arrays live in global memory; reading them through
 // global-address-space pointers gives global (not flat) loads, whose waits do
