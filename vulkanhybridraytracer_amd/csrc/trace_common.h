@@ -76,18 +76,13 @@ __device__ __forceinline__ gf4 gptr(const void* p) { return (gf4)(p); }
 // (far first, so the nearer ones pop first) and the nearest is returned. With
 // no hit the next entry is popped (DONE when the stack is empty). The stack
 // holds STACK_LDS entries per lane in LDS and the rest in `spill`, CAP in
-// all; a push beyond CAP sets the overflow flag. With TOP_AHEAD the stack top
-// is read while the node loads are in flight.
-template <uint32_t CAP, bool TOP_AHEAD = true>
+// all; a push beyond CAP sets the overflow flag.
+template <uint32_t CAP>
 __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur, const RayP& r, float tmin,
                                               float tmax, uint32_t* lds, uint32_t* spill, int& sp, Cnt& c) {
     const gf4 np = gptr(nodes4 + cur);
     const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
     const uint4 cd = ((gu4)np)[6];
-    // The stack top, read while the node loads are in flight: a node with no
-    // hit child then pops it without a second round trip. (Any-hit loops gain;
-    // the divergent closest-hit consumer loses more to the extra instructions.)
-    const uint32_t top = (TOP_AHEAD && sp > 0 && sp <= (int)STACK_LDS) ? lds[(sp - 1) * BLOCK] : DONE;
     const float inf = __uint_as_float(0x7f800000u);
     float t0, t1, t2, t3;
     const bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, r, tmin, tmax, t0) && cd.x != EMPTY4;
@@ -105,8 +100,7 @@ __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur
     if (nh == 0u) {
         if (sp == 0) return DONE;
         --sp;
-        if (sp >= (int)STACK_LDS) return spill[sp - STACK_LDS];
-        return TOP_AHEAD ? top : lds[sp * BLOCK];
+        return sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
     }
     auto push = [&](uint32_t code) {
         if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
@@ -137,7 +131,7 @@ __device__ bool traverse(const DevScene& S, const RayP& r, float tmin, float tma
         float bu = 0.0f, bv = 0.0f;
         for (;;) {
             if (cur < LEAF_BIT) {
-                cur = wide_step<MAX_STACK4, ANY>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
+                cur = wide_step<MAX_STACK4>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
                 continue;
             }
             if (cur == DONE) break;
@@ -492,7 +486,7 @@ __device__ void consume_rays(const DevScene& S, uint32_t n, uint32_t* next, uint
             continue;
         }
         while (!COUNT && cur < LEAF_BIT) {
-            cur = wide_step<STACK_CAP, ANY>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
+            cur = wide_step<STACK_CAP>(S.nodes4, cur, r, tmin, tbest, lds, spill, sp, c);
             // any hit: postpone the first leaf found and keep walking until
             // every lane still in this loop holds one. Closest hit: leaves in
             // walk order, so each one can shrink tbest before the next boxes
