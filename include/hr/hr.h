@@ -310,7 +310,9 @@ hr_status hr_scene_export(const hr_scene* scene, uint32_t which, void* dst_host,
 hr_status hr_gbuffer_primary(hr_scene* scene, const hr_camera* cam, const hr_gbuffer* gb,
                              hr_stats* stats);
 
-/* shadow: device float[nl][H][W], fraction of unoccluded samples per light. */
+/* shadow: device float[nl][H][W], fraction of unoccluded samples per light.
+ * nl may be 0 (lights may then be NULL), as the reference's light buffer may be
+ * empty (shadowRaygen.rgen:52): nothing is traced or written. */
 hr_status hr_trace_shadows(hr_scene* scene, const hr_gbuffer* gb,
                            const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
                            const hr_camera* cam, const hr_trace_params* params,
@@ -319,16 +321,25 @@ hr_status hr_trace_shadows(hr_scene* scene, const hr_gbuffer* gb,
 /* lit: device float4[H][W] final colour (imageStore, hybridRaygen.rgen:186,
  * unquantised); refl: device float4[H][W], the multiplicative bounce factor
  * averaged over refl_spp paths (1 where nothing is traced), so that
- * lit = direct * refl + ambient + emissive for non-background pixels. */
+ * lit = direct * refl + ambient + emissive for non-background pixels.
+ * With nl == 0 the direct term is 0 and no bounce is traced (the reference
+ * bounces a zero direction, hybridRaygen.rgen:66-67, whose colour multiplies a
+ * zero rayColor): lit = ambient + emissive, refl = 1. */
 hr_status hr_shade_hybrid(hr_scene* scene, const hr_gbuffer* gb,
                           const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
                           const hr_camera* cam, const hr_trace_params* params,
                           const float* shadow, float* lit, float* refl, hr_stats* stats);
 
-/* Bare ray queries over device arrays: any_hit != 0 stops at the first hit
- * (gl_RayFlagsTerminateOnFirstHitEXT); otherwise the closest hit, ties on t
- * broken by the smaller triangle id. */
-hr_status hr_trace_rays(hr_scene* scene, const hr_ray* rays, uint32_t n, int any_hit,
+/* Bare ray queries over device arrays. flags: HR_TRACE_ANY_HIT stops at the
+ * first hit (gl_RayFlagsTerminateOnFirstHitEXT); otherwise the closest hit,
+ * ties on t broken by the smaller triangle id. HR_TRACE_COUNT_VISITS walks the
+ * binary nodes in the oracle's near-first order and counts node visits and
+ * triangle tests into stats (a slower kernel); without it the 4-wide nodes
+ * are walked. Both give the same hits. stats (may be NULL) receives the
+ * counts and the time of whichever kernel ran. */
+#define HR_TRACE_ANY_HIT 1u
+#define HR_TRACE_COUNT_VISITS 2u
+hr_status hr_trace_rays(hr_scene* scene, const hr_ray* rays, uint32_t n, int flags,
                         hr_hit* hits, hr_stats* stats);
 
 /* ---- batches of frames (no counterpart in the reference, which renders one
@@ -358,7 +369,11 @@ hr_status hr_shade_hybrid_batch(hr_context* ctx, const hr_frame_desc* frames, ui
 /* A frame of a batch (shadows, then shading) captured once into a hipGraph
  * and replayed by hr_graph_launch. The graph owns its device copies of the
  * frames' parameters and its own workspace; the frames' scenes and buffers
- * must outlive it. */
+ * must outlive it. The graph holds the scenes' device addresses as they were
+ * at capture: after hr_scene_set_materials, hr_scene_set_textures or
+ * hr_scene_set_env on one of its scenes, hr_graph_launch returns HR_E_INVALID
+ * without launching (capture again). hr_scene_refit keeps the addresses, so a
+ * graph replays refitted scenes. */
 typedef struct hr_graph hr_graph;
 hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, hr_graph** out);
 hr_status hr_graph_launch(hr_graph* graph, hr_stats* stats);
@@ -371,7 +386,10 @@ void hr_graph_destroy(hr_graph* graph);
  *                  height 64 / tile_w): one wave's 64 groups
  *  wave_budget_mb  reflection-queue workspace per pixel slab, MiB (>= 64;
  *                  default 40960: a 1080p frame at 64 samples per pixel is one
- *                  slab) */
+ *                  slab). The workspace is allocated when a frame first needs
+ *                  it and kept by the context; when the device cannot hold it,
+ *                  the budget is halved (down to 64 MiB) and the frame runs in
+ *                  more, smaller slabs, with the same results. */
 typedef struct hr_tuning {
     uint32_t blocks_per_cu;
     uint32_t tile_w;

@@ -1717,7 +1717,10 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
                 }
             }
             v3 rfac = mk(1, 1, 1);
-            if ((mode == 3 || mode == 4) && f.max_depth > 0) {
+            // hybridRaygen.rgen:66-67, 125/144: the bounce direction is set in
+            // the light loop; with no lights it stays (0,0,0) and the bounce's
+            // colour multiplies rayColor = 0 (decision: no bounce is traced)
+            if ((mode == 3 || mode == 4) && f.max_depth > 0 && nl > 0) {
                 uint32_t spp = pixel_spp(f, mode, roughness);
                 v3 acc = mk(0, 0, 0);
                 for (uint32_t sI = 0; sI < spp; ++sI) {

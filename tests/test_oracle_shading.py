@@ -195,3 +195,34 @@ def test_rect_light_stratified_counts(oracle_lib, samples):
     lit, refl, c2 = os_.shade_hybrid(gb, sh)
     assert np.isfinite(lit).all()
     assert c2["reflection_rays"] > 0
+
+
+def test_zero_lights_ambient_only(oracle_lib):
+    """No lights (an empty light buffer, hybridRaygen.rgen:73): no shadow or
+    bounce ray is traced, refl = 1, and lit is the IBL ambient term
+    (1 - F(NdotV)) (1 - metallic) albedo^2.2 irradiance (hybridRaygen.rgen:
+    175-180) + emissive, here against float64 under a constant irradiance map."""
+    c = np.array([0.5, 0.6, 0.7], np.float32)
+    sc = _plane_sphere(mode_ground=3, sphere=True, env_const=c)
+    sc.lights = np.zeros((0, 12), np.float32)
+    os_ = oracle_lib.OracleScene(sc)
+    gb, _ = os_.gbuffer()
+    sh, cnt = os_.trace_shadows(gb)
+    assert sh.shape[0] == 0 and cnt["shadow_rays"] == 0
+    lit, refl, c2 = os_.shade_hybrid(gb, sh)
+    assert c2["reflection_rays"] == 0 and c2["hit_shadow_rays"] == 0
+    geo = gb["normal"][..., 3] == 1
+    assert np.all(refl[geo] == 1.0)
+    P = gb["position"][geo][:, :3].astype(np.float64)
+    N = gb["normal"][geo][:, :3].astype(np.float64)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    V = np.array([0, 12, 18.0]) - P
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    ndv = np.clip(np.sum(N * V, 1), 0, 1)
+    alb = gb["albedo"][geo][:, :3].astype(np.float64) ** 2.2
+    metal = gb["material"][geo][:, 2].astype(np.float64)[:, None]
+    F0 = 0.04 * (1 - metal) + alb * metal
+    F = F0 + (1 - F0) * ((1 - ndv) ** 5)[:, None]
+    want = (1 - F) * (1 - metal) * alb * c[None, :].astype(np.float64) + gb["emissive"][geo][:, :3]
+    assert (metal == 0).sum() > 50 and (metal == 1).sum() > 50  # both the sphere and the mirror floor
+    assert np.allclose(lit[geo][:, :3], want, rtol=1e-5, atol=1e-6)

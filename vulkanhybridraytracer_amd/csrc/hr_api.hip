@@ -36,6 +36,10 @@ struct hr_scene {
     TexInfo* d_tex = nullptr;
     uint32_t ntex = 0;
     int64_t max_tex = -1;          // largest texture index the materials use
+    // bumped by every setter that replaces device buffers or what the
+    // materials index (materials, textures, environment); a captured graph
+    // refuses to replay a scene whose generation moved
+    uint64_t generation = 0;
     BuildBuffers bb{};
 };
 
@@ -51,6 +55,17 @@ hr_status scene_alloc(hr_scene* s, T** p, size_t count) {
     s->allocs.push_back(q);
     *p = static_cast<T*>(q);
     return HR_OK;
+}
+// releases one buffer of s->allocs (device idle: the caller synchronised)
+void scene_free(hr_scene* s, void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < s->allocs.size(); ++i) {
+        if (s->allocs[i] == p) {
+            (void)hipFree(p);
+            s->allocs.erase(s->allocs.begin() + (long)i);
+            return;
+        }
+    }
 }
 // transpose(inverse(mat3(model))) as cofactors / determinant, row-major
 // (hybridHit.rchit:50); evaluated as written (no contraction)
@@ -150,7 +165,7 @@ hr_status fill_frame(hr_context* ctx, FrameArgs& f, const hr_camera* cam, uint32
 }
 hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, const hr_light_shape* shapes,
                       uint32_t nl, const hr_trace_params* prm) {
-    if (!lights || nl == 0) return fail(ctx, HR_E_INVALID, "at least one light is required");
+    if (!lights && nl != 0) return fail(ctx, HR_E_INVALID, "lights is NULL");
     if (nl > HR_MAX_LIGHTS) return fail(ctx, HR_E_UNSUPPORTED, "%u lights exceed HR_MAX_LIGHTS", nl);
     if (!prm) return fail(ctx, HR_E_INVALID, "trace params are NULL");
     if (prm->shadow_samples == 0 || prm->shadow_samples > 4096)
@@ -228,7 +243,10 @@ hr_status frame_end(hr_context* ctx, hr_stats* stats, float* ms_slot) {
 // Workspace budget of one slab of the reflection pipeline; larger frames are
 // processed in slabs of pixels.
 
-hr_status ensure_work(hr_context* ctx, size_t bytes) {
+// Grows the context's workspace to `bytes`. With `oom` non-NULL an
+// out-of-memory failure is reported there (and cleared) instead of failing.
+hr_status ensure_work(hr_context* ctx, size_t bytes, bool* oom = nullptr) {
+    if (oom) *oom = false;
     if (bytes <= ctx->work_bytes) return HR_OK;
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->work) {
@@ -236,18 +254,28 @@ hr_status ensure_work(hr_context* ctx, size_t bytes) {
         ctx->work = nullptr;
         ctx->work_bytes = 0;
     }
-    HR_HIP(ctx, hipMalloc(&ctx->work, bytes));
+    hipError_t e = hipMalloc(&ctx->work, bytes);
+    if (e == hipErrorOutOfMemory && oom) {
+        (void)hipGetLastError();  // not sticky: clear it for the next call
+        ctx->work = nullptr;
+        *oom = true;
+        return HR_OK;
+    }
+    if (e != hipSuccess) {
+        ctx->work = nullptr;
+        return hip_fail(ctx, e, "hipMalloc workspace");
+    }
     ctx->work_bytes = bytes;
     return HR_OK;
 }
 
 // Layout of the reflection workspace for frame f: returns its size and, with
 // a base address, fills w. Frames larger than the budget run in pixel slabs.
-size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char* b) {
+size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char* b, uint32_t budget_mb) {
     uint32_t spp_max = f.spp > 1 ? f.spp : 1;
     size_t per_slot = wave_bytes_per_slot(f.nl);
     size_t npix = (size_t)f.width * f.height;
-    size_t budget = (size_t)ctx->tuning.wave_budget_mb << 20;
+    size_t budget = (size_t)budget_mb << 20;
     size_t slab = budget / (per_slot * spp_max + 32);
     if (slab < 256) slab = 256;
     slab = slab < npix ? slab : npix;
@@ -285,11 +313,27 @@ size_t wave_layout(const hr_context* ctx, const FrameArgs& f, WaveArgs& w, char*
     return slots >= (size_t(1) << 28) ? 0 : off;
 }
 
+// The reflection workspace budget a frame runs with: the tuned budget, halved
+// while the device cannot hold the workspace it needs (64 MiB at least).
+// Smaller budgets only cut the frame into more pixel slabs. `need` gives the
+// workspace bytes at a budget (0: the frame does not fit any slab).
+template <class Need>
+hr_status fit_budget(hr_context* ctx, Need need, uint32_t& budget_mb) {
+    budget_mb = ctx->tuning.wave_budget_mb;
+    for (;;) {
+        const size_t bytes = need(budget_mb);
+        if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+        bool oom = false;
+        HR_TRY(ensure_work(ctx, bytes, budget_mb > 64u ? &oom : nullptr));
+        if (!oom) return HR_OK;
+        budget_mb = budget_mb / 2u < 64u ? 64u : budget_mb / 2u;
+    }
+}
+
 hr_status wave_args(hr_context* ctx, const FrameArgs& f, WaveArgs& w) {
-    size_t bytes = wave_layout(ctx, f, w, nullptr);
-    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
-    HR_TRY(ensure_work(ctx, bytes));
-    wave_layout(ctx, f, w, static_cast<char*>(ctx->work));
+    uint32_t mb = 0;
+    HR_TRY(fit_budget(ctx, [&](uint32_t b) { return wave_layout(ctx, f, w, nullptr, b); }, mb));
+    wave_layout(ctx, f, w, static_cast<char*>(ctx->work), mb);
     return HR_OK;
 }
 
@@ -336,24 +380,25 @@ hr_status plan_frame(hr_context* ctx, const hr_frame_desc& d, FramePlan& fp, boo
     return HR_OK;
 }
 
+// The shadow job of one frame whose groups are numbered from `base`;
+// returns the frame's group count.
+uint32_t fill_job(const hr_context* ctx, const FramePlan& fp, uint32_t base, ShadowJob& j) {
+    const uint32_t tw = ctx->tuning.tile_w;
+    memset(&j, 0, sizeof(j));
+    j.S = dev_scene(fp.scene);
+    j.F = fp.f;
+    j.G = fp.g;
+    j.shadow = fp.shadow;
+    j.group_base = base;
+    j.tile_w = tw;
+    j.tiles_x = (fp.f.width + tw - 1u) / tw;
+    return shadow_groups(fp.f, tw);
+}
 // Shadow jobs of a batch (host side); returns the total group count.
 uint32_t make_jobs(const hr_context* ctx, const std::vector<FramePlan>& plans, std::vector<ShadowJob>& jobs) {
-    uint32_t tw = ctx->tuning.tile_w, th = 64u / tw;
     uint32_t base = 0;
     jobs.resize(plans.size());
-    for (size_t i = 0; i < plans.size(); ++i) {
-        ShadowJob& j = jobs[i];
-        memset(&j, 0, sizeof(j));
-        j.S = dev_scene(plans[i].scene);
-        j.F = plans[i].f;
-        j.G = plans[i].g;
-        j.shadow = plans[i].shadow;
-        j.group_base = base;
-        j.tile_w = tw;
-        j.tiles_x = (plans[i].f.width + tw - 1u) / tw;
-        (void)th;
-        base += shadow_groups(plans[i].f, tw);
-    }
+    for (size_t i = 0; i < plans.size(); ++i) base += fill_job(ctx, plans[i], base, jobs[i]);
     return base;
 }
 
@@ -367,10 +412,10 @@ struct SplPlan {
     size_t scratch = 0;
 };
 
-SplPlan plan_spl(const std::vector<FramePlan>& plans, uint32_t ngroups) {
+SplPlan plan_spl(const FramePlan* plans, size_t nplans, uint32_t ngroups) {
     SplPlan sp;
     uint32_t smax = 0;
-    for (size_t i = 0; i < plans.size(); ++i) {
+    for (size_t i = 0; i < nplans; ++i) {
         const FrameArgs& f = plans[i].f;
         const uint32_t S = f.samples;
         if (plans[i].count || S < 4u || S > 64u || (S & (S - 1u)) != 0u) return sp;
@@ -731,6 +776,7 @@ hr_status hr_scene_set_materials(hr_scene* s, const hr_material* materials, uint
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     HR_HIP(ctx, hipMemcpy(s->d_mats, materials, count * sizeof(hr_material), hipMemcpyHostToDevice));
     s->max_tex = max_texture(materials, count);
+    ++s->generation;
     return HR_OK;
 }
 
@@ -770,6 +816,7 @@ hr_status hr_scene_set_textures(hr_scene* s, const hr_texture* textures, uint32_
     s->d_texels = texels;
     s->d_tex = tex;
     s->ntex = count;
+    ++s->generation;
     return HR_OK;
 }
 
@@ -780,20 +827,36 @@ hr_status hr_scene_set_env(hr_scene* s, const hr_env* env) {
         !env->irr_height)
         return fail(ctx, HR_E_INVALID, "environment maps must be non-empty");
     HR_HIP(ctx, hipSetDevice(ctx->device));
-    size_t ns = (size_t)env->sky_width * env->sky_height, ni = (size_t)env->irr_width * env->irr_height;
-    if (ns != (size_t)s->sky_w * s->sky_h) {
-        HR_TRY(scene_alloc(s, &s->d_sky, ns));
-    }
-    if (ni != (size_t)s->irr_w * s->irr_h) {
-        HR_TRY(scene_alloc(s, &s->d_irr, ni));
-    }
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    HR_HIP(ctx, hipMemcpy(s->d_sky, env->skybox, ns * sizeof(float4), hipMemcpyHostToDevice));
-    HR_HIP(ctx, hipMemcpy(s->d_irr, env->irradiance, ni * sizeof(float4), hipMemcpyHostToDevice));
+    size_t ns = (size_t)env->sky_width * env->sky_height, ni = (size_t)env->irr_width * env->irr_height;
+    // a map of another size gets a new buffer; the old one is released once
+    // the new contents are in place
+    float4* sky = s->d_sky;
+    float4* irr = s->d_irr;
+    if (ns != (size_t)s->sky_w * s->sky_h) HR_TRY(scene_alloc(s, &sky, ns));
+    if (ni != (size_t)s->irr_w * s->irr_h) {
+        hr_status st = scene_alloc(s, &irr, ni);
+        if (st != HR_OK) {
+            if (sky != s->d_sky) scene_free(s, sky);
+            return st;
+        }
+    }
+    hipError_t e = hipMemcpy(sky, env->skybox, ns * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(irr, env->irradiance, ni * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (sky != s->d_sky) scene_free(s, sky);
+        if (irr != s->d_irr) scene_free(s, irr);
+        return hip_fail(ctx, e, "environment upload");
+    }
+    if (sky != s->d_sky) scene_free(s, s->d_sky);
+    if (irr != s->d_irr) scene_free(s, s->d_irr);
+    s->d_sky = sky;
+    s->d_irr = irr;
     s->sky_w = env->sky_width;
     s->sky_h = env->sky_height;
     s->irr_w = env->irr_width;
     s->irr_h = env->irr_height;
+    ++s->generation;
     return HR_OK;
 }
 
@@ -902,7 +965,7 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     std::vector<ShadowJob> jobs;
     uint32_t ngroups = make_jobs(ctx, plans, jobs);
     HR_TRY(ensure_jobs(ctx, 1));
-    const SplPlan sp = plan_spl(plans, ngroups);
+    const SplPlan sp = plan_spl(plans.data(), plans.size(), ngroups);
     if (sp.use) HR_TRY(ensure_work(ctx, sp.scratch));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
@@ -932,13 +995,16 @@ hr_status hr_shade_hybrid(hr_scene* s, const hr_gbuffer* gb, const hr_light* lig
     return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 
-hr_status hr_trace_rays(hr_scene* s, const hr_ray* rays, uint32_t n, int any_hit, hr_hit* hits, hr_stats* stats) {
+hr_status hr_trace_rays(hr_scene* s, const hr_ray* rays, uint32_t n, int flags, hr_hit* hits, hr_stats* stats) {
     if (!s) return HR_E_INVALID;
     hr_context* ctx = s->ctx;
     if (n && (!rays || !hits)) return fail(ctx, HR_E_INVALID, "rays/hits pointer is NULL");
+    if ((uint32_t)flags & ~(HR_TRACE_ANY_HIT | HR_TRACE_COUNT_VISITS))
+        return fail(ctx, HR_E_INVALID, "unknown trace flags 0x%x", (unsigned)flags);
+    const int any_hit = ((uint32_t)flags & HR_TRACE_ANY_HIT) ? 1 : 0;
+    const bool count = ((uint32_t)flags & HR_TRACE_COUNT_VISITS) != 0;
     HR_TRY(frame_begin(ctx, stats != nullptr));
-    HR_HIP(ctx, launch_trace_rays(dev_scene(s), rays, n, any_hit, hits, ctx->d_counters, stats != nullptr,
-                                  ctx->stream));
+    HR_HIP(ctx, launch_trace_rays(dev_scene(s), rays, n, any_hit, hits, ctx->d_counters, count, ctx->stream));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -975,21 +1041,21 @@ hr_status plan_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, b
 
 // Kernels of a batch's shading pass, frames one after another on a shared
 // workspace at base (laid out per frame).
-hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base) {
+hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base, uint32_t budget_mb) {
     for (const FramePlan& fp : plans) {
         WaveArgs w;
-        wave_layout(ctx, fp.f, w, base);
+        wave_layout(ctx, fp.f, w, base, budget_mb);
         HR_HIP(ctx, launch_shade(dev_scene(fp.scene), fp.f, fp.g, fp.shadow, reinterpret_cast<float4*>(fp.lit),
                                  reinterpret_cast<float4*>(fp.refl), ctx->d_counters, fp.count, ctx->stream, w));
     }
     return HR_OK;
 }
 
-size_t batch_work_bytes(hr_context* ctx, const std::vector<FramePlan>& plans) {
+size_t batch_work_bytes(hr_context* ctx, const std::vector<FramePlan>& plans, uint32_t budget_mb) {
     size_t m = 0;
     for (const FramePlan& fp : plans) {
         WaveArgs w;
-        size_t b = wave_layout(ctx, fp.f, w, nullptr);
+        size_t b = wave_layout(ctx, fp.f, w, nullptr, budget_mb);
         if (b == 0) return 0;
         m = b > m ? b : m;
     }
@@ -1005,6 +1071,8 @@ struct hr_graph {
     ShadowJob* d_jobs = nullptr;
     uint32_t* d_next = nullptr;
     void* work = nullptr;
+    // the captured scenes and their generations at capture
+    std::vector<std::pair<const hr_scene*, uint64_t>> scenes;
 };
 
 extern "C" {
@@ -1018,7 +1086,7 @@ hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, u
     bool count = false;
     for (auto& fp : plans) count = count || fp.count;
     HR_TRY(ensure_jobs(ctx, n));
-    const SplPlan sp = plan_spl(plans, ngroups);
+    const SplPlan sp = plan_spl(plans.data(), plans.size(), ngroups);
     if (sp.use) HR_TRY(ensure_work(ctx, sp.scratch));
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
@@ -1031,11 +1099,10 @@ hr_status hr_shade_hybrid_batch(hr_context* ctx, const hr_frame_desc* frames, ui
     if (!ctx) return HR_E_INVALID;
     std::vector<FramePlan> plans;
     HR_TRY(plan_batch(ctx, frames, n, true, plans));
-    size_t bytes = batch_work_bytes(ctx, plans);
-    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
-    HR_TRY(ensure_work(ctx, bytes));
+    uint32_t mb = 0;
+    HR_TRY(fit_budget(ctx, [&](uint32_t b) { return batch_work_bytes(ctx, plans, b); }, mb));
     HR_TRY(frame_begin(ctx, stats != nullptr));
-    HR_TRY(enqueue_shading(ctx, plans, static_cast<char*>(ctx->work)));
+    HR_TRY(enqueue_shading(ctx, plans, static_cast<char*>(ctx->work), mb));
     return frame_end(ctx, stats, stats ? &stats->ms_shade : nullptr);
 }
 
@@ -1059,21 +1126,38 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     HR_TRY(plan_batch(ctx, frames, n, true, plans));
     std::vector<ShadowJob> jobs;
     uint32_t ngroups = make_jobs(ctx, plans, jobs);
-    size_t bytes = batch_work_bytes(ctx, plans);
-    if (bytes == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
     bool count = false;
     for (auto& fp : plans) count = count || fp.count;
     // the shadow pass's scratch and the shading workspace share g->work: the
     // passes run one after the other on the stream
-    const SplPlan sp = plan_spl(plans, ngroups);
-    if (sp.use && sp.scratch > bytes) bytes = sp.scratch;
+    const SplPlan sp = plan_spl(plans.data(), plans.size(), ngroups);
+    auto need = [&](uint32_t b) {
+        size_t bytes = batch_work_bytes(ctx, plans, b);
+        return (bytes != 0 && sp.use && sp.scratch > bytes) ? sp.scratch : bytes;
+    };
+    uint32_t mb = ctx->tuning.wave_budget_mb;
+    if (need(mb) == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
     HR_HIP(ctx, hipSetDevice(ctx->device));
     HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hr_graph* g = new hr_graph();
     g->ctx = ctx;
+    for (const FramePlan& fp : plans) g->scenes.emplace_back(fp.scene, fp.scene->generation);
     hipError_t e = hipMalloc(&g->d_jobs, n * sizeof(ShadowJob));
     if (e == hipSuccess) e = hipMalloc(&g->d_next, BAND_COUNTER_BYTES);
-    if (e == hipSuccess) e = hipMalloc(&g->work, bytes);
+    // the workspace at the tuned budget, halved while the device cannot hold it
+    while (e == hipSuccess) {
+        const size_t bytes = need(mb);
+        if (bytes == 0) {
+            hr_graph_destroy(g);
+            return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+        }
+        e = hipMalloc(&g->work, bytes);
+        if (e != hipErrorOutOfMemory || mb <= 64u) break;
+        (void)hipGetLastError();
+        g->work = nullptr;
+        e = hipSuccess;
+        mb = mb / 2u < 64u ? 64u : mb / 2u;
+    }
     if (e == hipSuccess) e = hipMemcpy(g->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         hr_graph_destroy(g);
@@ -1090,7 +1174,7 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
     if (st == HR_OK)
         st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work));
-    if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work));
+    if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work), mb);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);
     g->graph = graph;
@@ -1110,6 +1194,10 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
 hr_status hr_graph_launch(hr_graph* g, hr_stats* stats) {
     if (!g) return HR_E_INVALID;
     hr_context* ctx = g->ctx;
+    for (size_t i = 0; i < g->scenes.size(); ++i)
+        if (g->scenes[i].first->generation != g->scenes[i].second)
+            return fail(ctx, HR_E_INVALID, "frame %zu: its scene's materials, textures or environment changed "
+                        "since capture; capture the graph again", i);
     HR_HIP(ctx, hipSetDevice(ctx->device));
     if (stats) HR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     HR_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
@@ -1121,7 +1209,8 @@ hr_status hr_filter_shadows(hr_context* ctx, uint32_t width, uint32_t height, ui
     if (!ctx) return HR_E_INVALID;
     if (width == 0 || height == 0 || width > 16384 || height > 16384)
         return fail(ctx, HR_E_INVALID, "frame size %ux%u out of range", width, height);
-    if (nl == 0 || nl > HR_MAX_LIGHTS) return fail(ctx, HR_E_INVALID, "light_count %u out of range", nl);
+    if (nl > HR_MAX_LIGHTS) return fail(ctx, HR_E_INVALID, "light_count %u out of range", nl);
+    if (nl == 0) return HR_OK;  // nothing to filter
     if (!shadow || !history) return fail(ctx, HR_E_INVALID, "shadow and history must be device pointers");
     if (frame <= 0 && !motion) return fail(ctx, HR_E_INVALID, "motion vectors are needed when frame <= 0");
     HR_HIP(ctx, hipSetDevice(ctx->device));

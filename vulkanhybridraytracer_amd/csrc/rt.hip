@@ -524,7 +524,7 @@ hipError_t launch_rt_wavefront(const DevScene& s, const FrameArgs& f, float4* ac
         for (uint32_t depth = 0; depth < f.max_depth; ++depth) {
             k_rt_reset<<<1, 64, 0, st>>>(w, depth, next_item);
             k_rt_closest<<<nbT, BLOCK, 0, st>>>(s, f, w, depth, counters);
-            k_rt_clear_bits<<<nbW, 256, 0, st>>>(w, f.nl, lg2s);
+            if (nbW) k_rt_clear_bits<<<nbW, 256, 0, st>>>(w, f.nl, lg2s);
             k_rt_shadows<<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
             k_rt_shade<<<nbT, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth);
         }

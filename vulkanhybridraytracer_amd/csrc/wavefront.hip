@@ -300,7 +300,9 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
             v3 Fa = FresnelSchlick(NdotV, F0);
             v3 kD = scl(sub(mk(1.0f, 1.0f, 1.0f), Fa), 1.0f - metallic);
             v3 ambient = mul(mul(kD, albedo), irradiance);
-            bool traced = (mode == 3 || mode == 4) && F.max_depth > 0;
+            // no lights: the reference bounces a zero direction (set only in
+            // its light loop) into a zero rayColor, so nothing is traced
+            bool traced = (mode == 3 || mode == 4) && F.max_depth > 0 && F.nl > 0;
             if (!traced) {
                 v3 fc = add(add(rayColor, ambient), emis);
                 lit[p] = make_float4(fc.x, fc.y, fc.z, 1.0f);

@@ -166,6 +166,10 @@ class Context:
     def sync(self):
         self.check(self.L.hr_synchronize(self.h))
 
+    def last_error(self):
+        """hr_last_error: the message of the context's last failure (bytes)"""
+        return self.L.hr_last_error(self.h)
+
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
 
@@ -280,18 +284,30 @@ class HybridScene:
             arr[i] = abi.hr_texture(t.ctypes.data, t.shape[1], t.shape[0])
         self.ctx.check(self.ctx.L.hr_scene_set_textures(self.h, arr, len(ts)))
 
+    def set_env(self, sky, irradiance):
+        """environment maps: (height, width, 4) float32 equirect skybox and
+        irradiance (hr_scene_set_env)"""
+        sky = np.ascontiguousarray(sky, dtype=np.float32)
+        irr = np.ascontiguousarray(irradiance, dtype=np.float32)
+        env = abi.hr_env(sky.ctypes.data, sky.shape[1], sky.shape[0], irr.ctypes.data, irr.shape[1], irr.shape[0])
+        self.ctx.check(self.ctx.L.hr_scene_set_env(self.h, C.byref(env)))
+
     def set_materials(self, materials):
         m = np.ascontiguousarray(materials, dtype=np.float32)
         self.ctx.check(self.ctx.L.hr_scene_set_materials(self.h, m.ctypes.data, m.shape[0]))
 
-    def trace_rays(self, rays, any_hit=False, stats=False):
+    def trace_rays(self, rays, any_hit=False, stats=False, count=None):
+        """Bare ray queries (hr_trace_rays). count (default: stats) walks the
+        binary nodes and counts node visits and triangle tests."""
+        count = stats if count is None else count
         rays = np.ascontiguousarray(rays, dtype=abi.RAY_DTYPE)
         n = len(rays)
         rb = self.ctx.alloc(max(rays.nbytes, 32))
         hb = self.ctx.alloc(max(n * 16, 16))
         rb.upload(rays)
         st = abi.hr_stats() if stats else None
-        self.ctx.check(self.ctx.L.hr_trace_rays(self.h, rb.ptr, n, int(any_hit), hb.ptr,
+        flags = (abi.HR_TRACE_ANY_HIT if any_hit else 0) | (abi.HR_TRACE_COUNT_VISITS if count else 0)
+        self.ctx.check(self.ctx.L.hr_trace_rays(self.h, rb.ptr, n, flags, hb.ptr,
                                                 C.byref(st) if st is not None else None))
         hits = hb.download(n, abi.HIT_DTYPE)
         return (hits, st.as_dict()) if stats else hits
