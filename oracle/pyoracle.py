@@ -15,6 +15,33 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_hr.so")
 
 
+# A one-GPU box gives a process a share of 16 host threads, whatever the
+# machine has; the oracle's row workers never exceed that share.
+HOST_THREAD_SHARE = 16
+
+
+def default_threads():
+    """Worker threads for the oracle: the CPUs this process may run on, at
+    most HOST_THREAD_SHARE."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(HOST_THREAD_SHARE, n))
+
+
+def cpu_model():
+    """The host CPU's model name from /proc/cpuinfo (None when unreadable)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def build(force=False):
     """Compile the oracle with its Makefile (g++)."""
     cmd = ["make", "-C", HERE] + (["-B"] if force else [])
@@ -99,9 +126,9 @@ def _p(a):
 class OracleScene:
     """CPU LBVH scene built from a scenes.Scene (same inputs as hr_scene_build)."""
 
-    def __init__(self, scene, leaf_max=2, threads=0):
+    def __init__(self, scene, leaf_max=2, threads=None):
         self.scene = scene
-        self.threads = threads
+        self.threads = default_threads() if threads is None else threads
         self.packed = abi.PackedScene(scene)
         h = C.c_void_p()
         L = lib()

@@ -159,8 +159,20 @@ def test_graph_refuses_replay_after_setters(gpu_ctx):
     g.close()
 
 
+def _device_free_bytes():
+    """hipMemGetInfo of the HIP runtime libhr.so runs on (the device the
+    library last made current on this thread)."""
+    from vulkanhybridraytracer_amd import hr
+    out = subprocess.check_output(["ldd", hr.LIB_PATH]).decode()
+    path = next((ln.split("=>")[1].split()[0] for ln in out.splitlines() if "libamdhip64" in ln), None)
+    assert path, "libhr.so does not link libamdhip64"
+    hip = C.CDLL(path)
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
 def test_env_reupload_does_not_leak(gpu_ctx):
-    import torch
     hr = _hr()
     sc = scenes.make_small(width=32, height=18, spheres=2)
     hs = hr.HybridScene(gpu_ctx, sc)
@@ -168,11 +180,11 @@ def test_env_reupload_does_not_leak(gpu_ctx):
     small = scenes.make_env(1024, 512, 512, 256)
     hs.set_env(*big)
     gpu_ctx.sync()
-    free0 = torch.cuda.mem_get_info(0)[0]
+    free0 = _device_free_bytes()
     for k in range(40):
         hs.set_env(*(small if k % 2 == 0 else big))
     gpu_ctx.sync()
-    free1 = torch.cuda.mem_get_info(0)[0]
+    free1 = _device_free_bytes()
     # a leak would keep 40 x ~25 MiB; allow allocator noise
     assert free0 - free1 < (200 << 20), (free0 - free1) / 2**20
 

@@ -40,7 +40,11 @@ def test_filter_bit_exact(gpu_ctx, W, H, nl):
     gpu_ctx.check(L.hr_filter_shadows(gpu_ctx.h, W, H, nl, 3, sb.ptr, None, hb.ptr))
     assert np.array_equal(hb.download((nl, H, W)), T.filter_shadows(shadow, hist, 3))
     assert L.hr_filter_shadows(gpu_ctx.h, W, H, nl, 0, sb.ptr, None, hb.ptr) != 0
-    assert L.hr_filter_shadows(gpu_ctx.h, W, H, 0, 1, sb.ptr, None, hb.ptr) != 0
+    # no lights: nothing to filter, the history is left as it is
+    before = hb.download((nl, H, W))
+    assert L.hr_filter_shadows(gpu_ctx.h, W, H, 0, 1, sb.ptr, None, hb.ptr) == 0
+    assert np.array_equal(hb.download((nl, H, W)), before)
+    assert L.hr_filter_shadows(gpu_ctx.h, W, H, 17, 1, sb.ptr, None, hb.ptr) != 0  # above HR_MAX_LIGHTS
 
 
 def _view_proj(cam):
