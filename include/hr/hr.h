@@ -318,6 +318,24 @@ hr_status hr_trace_shadows(hr_scene* scene, const hr_gbuffer* gb,
                            const hr_camera* cam, const hr_trace_params* params,
                            float* shadow, hr_stats* stats);
 
+/* hr_trace_shadows without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc). Same
+ * arguments, checks and results (bit for bit) as hr_trace_shadows. Its scratch
+ * (the frame's shadow job, the work-band counters and the sample bit arrays)
+ * is one static device array of HR_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; a frame needing more returns HR_E_UNSUPPORTED before any
+ * device work (1920x1080 with 4 lights fits at every sample count up to 64,
+ * adaptive or not). Everything the pass needs zeroed is cleared by kernels on
+ * the context's stream. Calls that overlap on one GPU (other streams, contexts
+ * or threads) race on the shared array. Under graph capture pass stats = NULL
+ * (stats synchronise the stream). */
+#define HR_NOALLOC_SCRATCH_BYTES (128u << 20)
+hr_status hr_trace_shadows_noalloc(hr_scene* scene, const hr_gbuffer* gb,
+                                   const hr_light* lights, const hr_light_shape* shapes, uint32_t nl,
+                                   const hr_camera* cam, const hr_trace_params* params,
+                                   float* shadow, hr_stats* stats);
+
 /* lit: device float4[H][W] final colour (imageStore, hybridRaygen.rgen:186,
  * unquantised); refl: device float4[H][W], the multiplicative bounce factor
  * averaged over refl_spp paths (1 where nothing is traced), so that

@@ -22,6 +22,10 @@ struct hr_context {
     size_t work_bytes = 0;
     hr::ShadowJob* d_jobs = nullptr;  // grow-only device copy of the shadow jobs of a call
     uint32_t jobs_cap = 0;
+    // this device's address of the static scratch of hr_trace_shadows_noalloc
+    // (null when hipGetSymbolAddress failed: noalloc_err)
+    void* noalloc_scratch = nullptr;
+    hipError_t noalloc_err = hipSuccess;
     std::string err;
 };
 

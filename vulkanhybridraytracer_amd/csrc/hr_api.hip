@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "api_internal.h"
@@ -443,6 +444,35 @@ hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, 
     return HR_OK;
 }
 
+// ---- scratch of the allocation-free shadow pass ----------------------------------
+// One array per GPU, held until the process ends: [shadow job | band counters
+// | sample bits (| probe bits, refine list, count)]. In host code `scratch`
+// names a host-side stand-in, never the device array: kernels and copies use
+// the device address that hipGetSymbolAddress gives on each device.
+constexpr size_t NOALLOC_BYTES = HR_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+constexpr size_t NOALLOC_JOB_BYTES = (sizeof(ShadowJob) + 255u) / 256u * 256u;
+constexpr size_t NOALLOC_BITS_OFFSET = NOALLOC_JOB_BYTES + BAND_COUNTER_BYTES;
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with the
+// device current (the caller has made it current).
+hipError_t noalloc_scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -468,6 +498,9 @@ hr_status hr_context_create(int device, hr_context** out) {
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
             ctx->cu_count = (uint32_t)prop.multiProcessorCount;
         ctx->resident_blocks = ctx->cu_count * ctx->tuning.blocks_per_cu;
+        // only hr_trace_shadows_noalloc needs it: a failure is kept, not fatal
+        ctx->noalloc_err = noalloc_scratch_address(device, &ctx->noalloc_scratch);
+        if (ctx->noalloc_err != hipSuccess) (void)hipGetLastError();
     }
     if (e != hipSuccess) {
         hr_context_destroy(ctx);
@@ -971,6 +1004,43 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, 1, ngroups, sp, count, ctx->d_next_group,
                            reinterpret_cast<uint32_t*>(ctx->work)));
+    return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
+}
+
+hr_status hr_trace_shadows_noalloc(hr_scene* s, const hr_gbuffer* gb, const hr_light* lights,
+                                   const hr_light_shape* shapes, uint32_t nl, const hr_camera* cam,
+                                   const hr_trace_params* params, float* shadow, hr_stats* stats) {
+    if (!s) return HR_E_INVALID;
+    hr_context* ctx = s->ctx;
+    if (!ctx->noalloc_scratch)
+        return fail(ctx, HR_E_DEVICE, "the allocation-free scratch is unavailable on device %d: %s", ctx->device,
+                    hipGetErrorString(ctx->noalloc_err));
+    HR_TRY(check_gbuffer(ctx, gb));
+    if (!shadow) return fail(ctx, HR_E_INVALID, "shadow output is NULL");
+    // as hr_trace_shadows, with the job on the stack instead of in vectors
+    FramePlan fp;
+    fp.scene = s;
+    HR_TRY(fill_frame(ctx, fp.f, cam, gb->width, gb->height));
+    HR_TRY(fill_lights(ctx, fp.f, lights, shapes, nl, params));
+    fp.g = gview(gb);
+    fp.shadow = shadow;
+    fp.lit = nullptr;
+    fp.refl = nullptr;
+    fp.count = (params->flags & HR_FLAG_COUNT_VISITS) != 0;
+    ShadowJob job;
+    const uint32_t ngroups = fill_job(ctx, fp, 0u, job);
+    const SplPlan sp = plan_spl(&fp, 1, ngroups);
+    const size_t need = NOALLOC_BITS_OFFSET + (sp.use ? sp.scratch : 0);
+    if (need > NOALLOC_BYTES)
+        return fail(ctx, HR_E_UNSUPPORTED, "the frame needs %zu bytes of scratch, above HR_NOALLOC_SCRATCH_BYTES (%zu)",
+                    need, NOALLOC_BYTES);
+    char* base = static_cast<char*>(ctx->noalloc_scratch);
+    ShadowJob* d_job = reinterpret_cast<ShadowJob*>(base);
+    uint32_t* d_bands = reinterpret_cast<uint32_t*>(base + NOALLOC_JOB_BYTES);
+    uint32_t* d_bits = reinterpret_cast<uint32_t*>(base + NOALLOC_BITS_OFFSET);
+    HR_TRY(frame_begin(ctx, stats != nullptr));
+    HR_HIP(ctx, launch_store_job(job, d_job, ctx->stream));
+    HR_TRY(enqueue_shadows(ctx, d_job, 1, ngroups, sp, fp.count, d_bands, d_bits));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 

@@ -152,6 +152,8 @@ struct ShadowJob {
     uint32_t _pad;
 };
 uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w);
+// writes `job` to device memory at dst (16-byte aligned) from a kernel argument
+hipError_t launch_store_job(const ShadowJob& job, ShadowJob* dst, hipStream_t st);
 // Frames with power-of-two sample counts in 4..64 (with adaptive sampling the
 // penumbra counts), all adaptive or none: one lane per (group, sample) item,
 // 2^lg2s = the largest count. scratch: device memory of

@@ -494,6 +494,15 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+// copies a shadow job passed by value (a kernel argument, so a captured
+// graph keeps its own copy) into device memory, one 32-bit word per lane
+static_assert(sizeof(ShadowJob) % 4 == 0, "ShadowJob is copied in 32-bit words");
+constexpr uint32_t JOB_WORDS = (uint32_t)(sizeof(ShadowJob) / 4);
+__global__ __launch_bounds__(256) void k_store_job(ShadowJob job, uint32_t* dst) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&job);
+    for (uint32_t i = threadIdx.x; i < JOB_WORDS; i += blockDim.x) dst[i] = src[i];
+}
+
 inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u) * ((f.height + 15u) / 16u); }
 
 
@@ -677,6 +686,10 @@ __global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __r
 
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t st) {
     k_zero_u64<<<(n + 63) / 64, 64, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+hipError_t launch_store_job(const ShadowJob& job, ShadowJob* dst, hipStream_t st) {
+    k_store_job<<<1, 256, 0, st>>>(job, reinterpret_cast<uint32_t*>(dst));
     return hipGetLastError();
 }
 hipError_t launch_gbuffer(const DevScene& s, const FrameArgs& f, GBufView gb, unsigned long long* counters, bool count,

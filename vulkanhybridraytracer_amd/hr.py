@@ -95,6 +95,8 @@ def load(path=LIB_PATH):
                                     C.POINTER(abi.hr_stats)]),
         "hr_trace_shadows": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
                                   C.POINTER(abi.hr_trace_params), vp, C.POINTER(abi.hr_stats)]),
+        "hr_trace_shadows_noalloc": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
+                                          C.POINTER(abi.hr_trace_params), vp, C.POINTER(abi.hr_stats)]),
         "hr_shade_hybrid": (st, [vp, C.POINTER(abi.hr_gbuffer), vp, vp, u32, C.POINTER(abi.hr_camera),
                                  C.POINTER(abi.hr_trace_params), vp, vp, vp, C.POINTER(abi.hr_stats)]),
         "hr_trace_rays": (st, [vp, vp, u32, i32, vp, C.POINTER(abi.hr_stats)]),
