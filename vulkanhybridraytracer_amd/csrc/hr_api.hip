@@ -433,11 +433,12 @@ SplPlan plan_spl(const FramePlan* plans, size_t nplans, uint32_t ngroups) {
 
 // Kernels of a shadow pass (device jobs already uploaded); scratch holds
 // sp.scratch bytes when sp.use.
+// (node_count: the 4-wide node slots of the first frame's scene; used for one frame)
 hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, uint32_t ngroups, const SplPlan& sp,
-                          bool count, uint32_t* next_group, uint32_t* scratch) {
+                          bool count, uint32_t* next_group, uint32_t* scratch, uint32_t node_count) {
     if (sp.use)
         HR_HIP(ctx, launch_shadows_spl(d_jobs, n, ngroups, sp.lg2s, sp.adaptive, ctx->d_counters, ctx->stream,
-                                       next_group, scratch, ctx->resident_blocks));
+                                       next_group, scratch, ctx->resident_blocks, node_count));
     else
         HR_HIP(ctx, launch_shadows(d_jobs, n, ngroups, ctx->d_counters, count, ctx->stream, next_group,
                                    ctx->resident_blocks));
@@ -1003,7 +1004,7 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, 1, ngroups, sp, count, ctx->d_next_group,
-                           reinterpret_cast<uint32_t*>(ctx->work)));
+                           reinterpret_cast<uint32_t*>(ctx->work), s->node_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1040,7 +1041,7 @@ hr_status hr_trace_shadows_noalloc(hr_scene* s, const hr_gbuffer* gb, const hr_l
     uint32_t* d_bits = reinterpret_cast<uint32_t*>(base + NOALLOC_BITS_OFFSET);
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, launch_store_job(job, d_job, ctx->stream));
-    HR_TRY(enqueue_shadows(ctx, d_job, 1, ngroups, sp, fp.count, d_bands, d_bits));
+    HR_TRY(enqueue_shadows(ctx, d_job, 1, ngroups, sp, fp.count, d_bands, d_bits, s->node_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1161,7 +1162,7 @@ hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, u
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, n, ngroups, sp, count, ctx->d_next_group,
-                           reinterpret_cast<uint32_t*>(ctx->work)));
+                           reinterpret_cast<uint32_t*>(ctx->work), plans[0].scene->node_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1243,7 +1244,8 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     hipError_t le = launch_zero_u64(ctx->d_counters, C_COUNT * C_STRIDE, ctx->stream);
     if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
     if (st == HR_OK)
-        st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work));
+        st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work),
+                             plans[0].scene->node_count);
     if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work), mb);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);

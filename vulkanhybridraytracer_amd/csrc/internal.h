@@ -159,9 +159,11 @@ hipError_t launch_store_job(const ShadowJob& job, ShadowJob* dst, hipStream_t st
 // 2^lg2s = the largest count. scratch: device memory of
 // shadows_spl_scratch() bytes.
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive);
+// node_count: the 4-wide node slots of a single frame's scene (to choose
+// 32-bit node addressing), ignored for several frames
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                               bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
-                              uint32_t* scratch, uint32_t resident_blocks);
+                              uint32_t* scratch, uint32_t resident_blocks, uint32_t node_count);
 // jobs_dev: njobs ShadowJob in device memory; next_group: BAND_COUNTER_BYTES
 // of device memory holding the SHADOW_QUEUES band counters, BAND_STRIDE words
 // apart (reset by the launch); resident_blocks: the persistent grid size

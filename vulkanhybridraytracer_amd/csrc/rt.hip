@@ -402,6 +402,7 @@ struct RtItems {
     }
 };
 
+template <bool OFF32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_rt_shadows(
     DevScene S, FrameArgs F, RtWork w, uint32_t lg2s, uint32_t nS, uint32_t depth, uint32_t* next_item,
     unsigned long long* counters) {
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     Cnt c = {};
     const uint32_t nh = w.cnt[2];
     RtItems src{F, w.hpos, w.hnrm, w.hpix, nh, lg2s, nS, depth, S.nodes4, S.tris, S.node_count};
-    item_loop(src, (F.nl * nh) << lg2s, next_item, w.bits, stk + threadIdx.x, c);
+    item_loop<OFF32>(src, (F.nl * nh) << lg2s, next_item, w.bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
 }
 
@@ -525,7 +526,10 @@ hipError_t launch_rt_wavefront(const DevScene& s, const FrameArgs& f, float4* ac
             k_rt_reset<<<1, 64, 0, st>>>(w, depth, next_item);
             k_rt_closest<<<nbT, BLOCK, 0, st>>>(s, f, w, depth, counters);
             if (nbW) k_rt_clear_bits<<<nbW, 256, 0, st>>>(w, f.nl, lg2s);
-            k_rt_shadows<<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
+            if ((uint64_t)s.node_count * sizeof(TNode4) <= 0xFFFFFFFFull)
+                k_rt_shadows<true><<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
+            else
+                k_rt_shadows<false><<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
             k_rt_shade<<<nbT, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth);
         }
         k_rt_end<<<nbP, BLOCK, 0, st>>>(f, w, nS, accum, frame_signed);

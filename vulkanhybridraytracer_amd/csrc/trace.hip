@@ -597,18 +597,37 @@ struct HybridItems {
     }
 };
 
-template <int MODE, bool MULTI>
+// OFF32 (one frame whose 4-wide node array is below 4 GiB): nodes are
+// addressed by 32-bit offsets from the wave-uniform array base.
+template <int MODE, bool MULTI, bool OFF32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
     uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
     unsigned long long* counters) {
+    static_assert(!(MULTI && OFF32), "frames of a batch have their own node arrays");
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
     if (MODE == 2) nitems = *list_count << lg2s;
     HybridItems<MODE, MULTI> src{jobs, njobs, MODE == 1 ? 2u : lg2s, list, jobs[0].S.nodes4, jobs[0].S.tris,
                                  jobs[0].S.node_count};
-    item_loop(src, nitems, next_item, bits, stk + threadIdx.x, c);
+    item_loop<OFF32>(src, nitems, next_item, bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
+}
+
+// launches k_shadows_spl<MODE, ...> for njobs frames
+template <int MODE>
+void launch_spl_mode(bool multi, bool off32, uint32_t nb, hipStream_t st, const ShadowJob* jobs, uint32_t njobs,
+                     uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits, const uint32_t* list,
+                     const uint32_t* list_count, unsigned long long* counters) {
+    if (multi)
+        k_shadows_spl<MODE, true, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
+                                                               list_count, counters);
+    else if (off32)
+        k_shadows_spl<MODE, false, true><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
+                                                               list_count, counters);
+    else
+        k_shadows_spl<MODE, false, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
+                                                                list_count, counters);
 }
 
 // shadow factor of every group from the bits of its samples
@@ -712,7 +731,9 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
 
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                               bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
-                              uint32_t* scratch, uint32_t resident_blocks) {
+                              uint32_t* scratch, uint32_t resident_blocks, uint32_t node_count) {
+    const bool multi = njobs > 1u;
+    const bool off32 = !multi && (uint64_t)node_count * sizeof(TNode4) <= 0xFFFFFFFFull;
     const uint64_t items = (uint64_t)ngroups << lg2s;
     if (items >= (1ull << 32)) return hipErrorInvalidValue;
     const uint32_t words = (uint32_t)((items + 31u) >> 5);
@@ -722,12 +743,8 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     if (!adaptive) {
         k_reset_u32<<<1, 64, 0, st>>>(next_item);
         k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-        if (njobs > 1u)
-            k_shadows_spl<0, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits,
-                                                         nullptr, nullptr, counters);
-        else
-            k_shadows_spl<0, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits,
-                                                          nullptr, nullptr, counters);
+        launch_spl_mode<0>(multi, off32, nb, st, jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits, nullptr,
+                           nullptr, counters);
         k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
@@ -741,21 +758,12 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(pwords + 255u) / 256u, 256, 0, st>>>(pbits, pwords);
     k_fill_bits<<<1, 64, 0, st>>>(count, 1u);
-    if (njobs > 1u)
-        k_shadows_spl<1, true><<<nbp, BLOCK, 0, st>>>(jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr,
-                                                      nullptr, counters);
-    else
-        k_shadows_spl<1, false><<<nbp, BLOCK, 0, st>>>(jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr,
-                                                       nullptr, counters);
+    launch_spl_mode<1>(multi, off32, nbp, st, jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr, nullptr,
+                       counters);
     k_shadow_decide<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, pbits, list, count);
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-    if (njobs > 1u)
-        k_shadows_spl<2, true><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count,
-                                                     counters);
-    else
-        k_shadows_spl<2, false><<<nb, BLOCK, 0, st>>>(jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count,
-                                                      counters);
+    launch_spl_mode<2>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, counters);
     k_shadow_refine_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, lg2s, bits, list, count);
     return hipGetLastError();
 }

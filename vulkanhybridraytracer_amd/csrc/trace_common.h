@@ -114,6 +114,101 @@ __device__ __forceinline__ uint32_t wide_step(const TNode4* nodes4, uint32_t cur
     return c0;
 }
 
+// Entry sp of a lane's stack (16 in LDS, the rest in `spill`): the LDS word is
+// read whatever sp is (an LDS read, not a generic load that waits on memory
+// too), and replaced from `spill` in the rare deep case.
+__device__ __forceinline__ uint32_t stack_at(const uint32_t* lds, const uint32_t* spill, int sp) {
+    uint32_t v = lds[min(sp, (int)STACK_LDS - 1) * BLOCK];
+    if (sp >= (int)STACK_LDS) v = spill[sp - STACK_LDS];
+    return v;
+}
+
+// Byte offsets in a 4-wide node of a ray's near x, y and z planes: the lo
+// planes where the direction component is positive, the hi planes where it is
+// negative (the inverse direction keeps the sign even for a zero component);
+// the far plane of each axis is at offset ^ 16.
+struct NearPlanes {
+    uint32_t x, y, z;
+};
+__device__ __forceinline__ NearPlanes near_planes(const RayP& r) {
+    return NearPlanes{r.idir.x < 0.0f ? 16u : 0u, r.idir.y < 0.0f ? 48u : 32u, r.idir.z < 0.0f ? 80u : 64u};
+}
+
+// One any-hit step over a 4-wide node, as wide_step but with each axis's
+// near and far planes loaded by the ray's direction signs. The slab distances
+// are those of box_hit (fma is monotone in the plane for a fixed inverse
+// direction, so the near plane's distance is the min of the two and the far
+// plane's the max): the same boxes are entered. An empty slot's inverted box
+// (lo = +inf, hi = -inf) then misses by itself: its near distance is +inf or
+// its far distance -inf (at most one of the two can be NaN), so no code check
+// is needed. OFF32: the node array is addressed by 32-bit byte offsets from
+// its (wave-uniform) base, which holds for arrays below 4 GiB.
+template <bool OFF32>
+__device__ __forceinline__ uint32_t wide_step_any(const TNode4* nodes4, uint32_t cur, const RayP& r,
+                                                  const NearPlanes& pl, float tmin, float tmax, uint32_t* lds,
+                                                  uint32_t* spill, int& sp, Cnt& c) {
+    float4 nx, fx, ny, fy, nz, fz;
+    uint4 cd;
+    if (OFF32) {
+        const char* b = reinterpret_cast<const char*>(nodes4);
+        const uint32_t o = cur << 7;
+        nx = *gptr(b + (o + pl.x));
+        fx = *gptr(b + (o + (pl.x ^ 16u)));
+        ny = *gptr(b + (o + pl.y));
+        fy = *gptr(b + (o + (pl.y ^ 16u)));
+        nz = *gptr(b + (o + pl.z));
+        fz = *gptr(b + (o + (pl.z ^ 16u)));
+        cd = *(gu4)gptr(b + (o + 96u));
+    } else {
+        const char* b = reinterpret_cast<const char*>(nodes4 + cur);
+        nx = *gptr(b + pl.x);
+        fx = *gptr(b + (pl.x ^ 16u));
+        ny = *gptr(b + pl.y);
+        fy = *gptr(b + (pl.y ^ 16u));
+        nz = *gptr(b + pl.z);
+        fz = *gptr(b + (pl.z ^ 16u));
+        cd = *(gu4)gptr(b + 96u);
+    }
+    auto slab = [&](float ax, float bx, float ay, float by, float az, float bz, float& tn) {
+        const float tnx = fmaf(ax, r.idir.x, -r.ood.x), tfx = fmaf(bx, r.idir.x, -r.ood.x);
+        const float tny = fmaf(ay, r.idir.y, -r.ood.y), tfy = fmaf(by, r.idir.y, -r.ood.y);
+        const float tnz = fmaf(az, r.idir.z, -r.ood.z), tfz = fmaf(bz, r.idir.z, -r.ood.z);
+        const float n = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        const float f = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+        tn = n;
+        return n <= f * 1.0000002384185791f;
+    };
+    const float inf = __uint_as_float(0x7f800000u);
+    float t0, t1, t2, t3;
+    const bool b0 = slab(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, t0);
+    const bool b1 = slab(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, t1);
+    const bool b2 = slab(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, t2);
+    const bool b3 = slab(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, t3);
+    float k0 = b0 ? t0 : inf, k1 = b1 ? t1 : inf, k2 = b2 ? t2 : inf, k3 = b3 ? t3 : inf;
+    uint32_t c0 = cd.x, c1 = cd.y, c2 = cd.z, c3 = cd.w;
+    sort_pair(k0, c0, k1, c1);
+    sort_pair(k2, c2, k3, c3);
+    sort_pair(k0, c0, k2, c2);
+    sort_pair(k1, c1, k3, c3);
+    sort_pair(k1, c1, k2, c2);
+    const uint32_t nh = (uint32_t)b0 + (uint32_t)b1 + (uint32_t)b2 + (uint32_t)b3;
+    if (nh == 0u) {
+        if (sp == 0) return DONE;
+        --sp;
+        return stack_at(lds, spill, sp);
+    }
+    auto push = [&](uint32_t code) {
+        if (sp < (int)STACK_LDS) lds[sp * BLOCK] = code;
+        else if (sp < (int)MAX_STACK4) spill[sp - STACK_LDS] = code;
+        else c.overflow = 1;
+        sp = min(sp + 1, (int)MAX_STACK4);
+    };
+    if (nh > 3u) push(c3);
+    if (nh > 2u) push(c2);
+    if (nh > 1u) push(c1);
+    return c0;
+}
+
 // Closest-hit (ANY = false) or any-hit traversal of one ray. The counting
 // variant walks the binary nodes in near-first order (the oracle's order and
 // counts); the timed variant walks the 4-wide nodes. Results are the same:
@@ -701,7 +796,7 @@ __device__ __forceinline__ void or_item_bits(uint32_t* bits, uint32_t item, bool
     }
 }
 
-template <class Src>
+template <bool OFF32, class Src>
 __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* next_item, uint32_t* bits,
                                           uint32_t* lds, Cnt& c) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -715,6 +810,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
     bool tracing = false, hit = false;
     uint32_t cur = DONE, leaf = 0u;
     RayP r;
+    NearPlanes pl{0u, 32u, 64u};
     float tmax = 0.0f;
     const float tmin = 0.001f;
     int sp = 0;
@@ -753,6 +849,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                 const int kind = src.setup(item, r, tmax, nodes4, tris, node_count, c);
                 or_item_bits(bits, item, kind == ITEM_LIT, slot);
                 if (kind != ITEM_TRACE) continue;
+                if (OFF32) pl = near_planes(r);
                 tracing = true;
                 hit = false;
                 sp = 0;
@@ -763,14 +860,17 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
             continue;
         }
         while (cur < LEAF_BIT) {
-            cur = wide_step<MAX_STACK4>(nodes4, cur, r, tmin, tmax, lds, spill, sp, c);
+            // per-lane 64-bit plane addresses cost more registers than the
+            // min/max they save: the sign-selected step only with 32-bit offsets
+            if (OFF32) cur = wide_step_any<true>(nodes4, cur, r, pl, tmin, tmax, lds, spill, sp, c);
+            else cur = wide_step<MAX_STACK4>(nodes4, cur, r, tmin, tmax, lds, spill, sp, c);
             if (leaf == 0u && cur >= LEAF_BIT && cur != DONE) {
                 leaf = cur;
                 if (sp == 0) {
                     cur = DONE;
                 } else {
                     --sp;
-                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                    cur = stack_at(lds, spill, sp);
                 }
             }
             if (__ballot(leaf == 0u) == 0ull || __popcll(__ballot(cur < LEAF_BIT)) < REFILL_MIN_ITEMS) break;
@@ -810,7 +910,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                     cur = DONE;
                 } else {
                     --sp;
-                    cur = sp < (int)STACK_LDS ? lds[sp * BLOCK] : spill[sp - STACK_LDS];
+                    cur = stack_at(lds, spill, sp);
                 }
             }
         }
