@@ -277,6 +277,19 @@ static inline v3 sampleDisk(const hr_light& light, v3 position, v3 L, float r1, 
     v3 target = add(add(add(position, L), scl(tangent, px)), scl(bitangent, py));
     return normalize(sub(target, position));
 }
+/* Planted mistakes (tests only): or_set_plant selects one deliberate error so
+ * that a test can show its independent check fails on it. 0 = none. */
+enum : int {
+    PLANT_NONE = 0,
+    PLANT_GLASS_ETA = 1,      /* refrEta = ior <-> 1 / ior swapped */
+    PLANT_GLASS_NORMAL = 2,   /* refrNormal not flipped for rays leaving the surface */
+    PLANT_FRESNEL_NDOTH = 3,  /* hybridRaygen.rgen:111 Fresnel of NdotH instead of HdotV */
+    PLANT_GGX_ALPHA = 4,      /* sampleGGX with a = roughness instead of roughness^2 */
+    PLANT_RECT_STRATA = 5,    /* sampleRect rows divided by n instead of m */
+    PLANT_TIR_REFRACT = 6     /* total internal reflection answered by refract() */
+};
+static int g_plant = PLANT_NONE;
+
 /* stratified point on a rectangle light: cell s of an n x m grid */
 static inline v3 sampleRect(v3 centre, const hr_light_shape& sh, v3 position, uint32_t s,
                             uint32_t S, float r1, float r2) {
@@ -284,7 +297,7 @@ static inline v3 sampleRect(v3 centre, const hr_light_shape& sh, v3 position, ui
     while ((n + 1) * (n + 1) <= S) ++n;
     uint32_t m = (S + n - 1) / n;
     float fu = ((float)(s % n) + r1) / (float)n - 0.5f;
-    float fv = ((float)(s / n) + r2) / (float)m - 0.5f;
+    float fv = ((float)(s / n) + r2) / (float)(g_plant == PLANT_RECT_STRATA ? n : m) - 0.5f;
     v3 eu = mk(sh.edge_u[0], sh.edge_u[1], sh.edge_u[2]);
     v3 ev = mk(sh.edge_v[0], sh.edge_v[1], sh.edge_v[2]);
     v3 target = add(add(centre, scl(eu, fu)), scl(ev, fv));
@@ -300,7 +313,7 @@ static inline void createCoordSystem(v3 N, v3& T, v3& B) {
 }
 /* GGX microfacet normal (Walter et al. 2007) around N with a = roughness^2 */
 static inline v3 sampleGGX(v3 N, float roughness, float r1, float r2) {
-    float a = roughness * roughness;
+    float a = g_plant == PLANT_GGX_ALPHA ? roughness : roughness * roughness;
     float cos2 = (1.0f - r1) / (1.0f + (a * a - 1.0f) * r1);
     float ct = sqrtf(cos2);
     float st = sqrtf(fmaxf(1.0f - cos2, 0.0f));
@@ -309,6 +322,40 @@ static inline v3 sampleGGX(v3 N, float roughness, float r1, float r2) {
     v3 T, B;
     createCoordSystem(N, T, B);
     return add(add(scl(T, st * cs), scl(B, st * sn)), scl(N, ct));
+}
+
+/* Glass (shading model 4) directions. Both shaders flip the normal and use
+ * 1 / ior as the ratio for rays leaving the surface (N.I > 0), the normal and
+ * ior for rays entering it, and reflect about N when the radicand
+ * 1 + eta^2 (cos^2 - 1) is negative. */
+static inline void glass_frame(v3 N, float NdotI, float ior, v3& refrNormal, float& refrEta) {
+    const bool out = NdotI > 0.0f;
+    refrNormal = (out != (g_plant == PLANT_GLASS_NORMAL)) ? neg(N) : N;
+    refrEta = (out != (g_plant == PLANT_GLASS_ETA)) ? 1.0f / ior : ior;
+}
+/* hybridRaygen.rgen:131-144: from the surface seen along V (towards the eye) */
+static inline v3 glass_raygen_dir(v3 N, v3 V, float ior) {
+    float cosAlpha = dot(N, V);
+    v3 I = neg(V);
+    float NdotI = dot(N, I);
+    v3 refrNormal;
+    float refrEta;
+    glass_frame(N, NdotI, ior, refrNormal, refrEta);
+    float radicand = 1.0f + refrEta * refrEta * (cosAlpha * cosAlpha - 1.0f);
+    if (radicand < 0.0f && g_plant != PLANT_TIR_REFRACT) return reflect(I, N);
+    return refract(I, refrNormal, refrEta);
+}
+/* hybridHit.rchit:158-173: at a hit reached along rd (gl_WorldRayDirectionEXT);
+ * the reflection takes rd as it is, the refraction its normalisation */
+static inline v3 glass_hit_dir(v3 N, v3 rd, float ior) {
+    v3 rdn = normalize(rd);
+    float ndv = dot(N, rdn);
+    v3 refrNormal;
+    float refrEta;
+    glass_frame(N, ndv, ior, refrNormal, refrEta);
+    float radicand = 1.0f + refrEta * refrEta * (ndv * ndv - 1.0f);
+    if (radicand < 0.0f && g_plant != PLANT_TIR_REFRACT) return reflect(rd, N);
+    return refract(rdn, refrNormal, refrEta);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1087,15 +1134,10 @@ static HitOut shade_hit(const or_scene& s, const Frame& f, v3 ro, v3 rd, const H
             direction = reflected;
             scatter = sc;
         } else if (mode == 4) {
-            float ior = mat.diffuse_ior[3];
-            float ndv = dot(N, rdn);
-            v3 refrNormal = ndv > 0.0f ? neg(N) : N;
-            float refrEta = ndv > 0.0f ? 1.0f / ior : ior;
             v3 add_t = (intensity > 0.0f) ? scl(mul(scl(scl(lc, intensity), attenuation), albedo), metallic)
                                           : scl(mul(irradiance, albedo), metallic);
             Lo = add(Lo, add_t);
-            float radicand = 1.0f + refrEta * refrEta * (ndv * ndv - 1.0f);
-            direction = radicand < 0.0f ? reflect(rd, N) : refract(rdn, refrNormal, refrEta);
+            direction = glass_hit_dir(N, rd, mat.diffuse_ior[3]);
             scatter = true;
         }
     }
@@ -1275,6 +1317,25 @@ void or_fresnel_schlick(float ct, const float* F0, float* out3) {
 void or_sample_disk(const hr_light* light, const float* pos, const float* L, float r1, float r2, float* d) {
     v3 v = sampleDisk(*light, ld3(pos), ld3(L), r1, r2);
     d[0] = v.x; d[1] = v.y; d[2] = v.z;
+}
+
+void or_set_plant(int plant) { g_plant = plant; }
+void or_sample_ggx(const float* N, float roughness, float r1, float r2, float* out) {
+    v3 d = sampleGGX(ld3(N), roughness, r1, r2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void or_sample_rect(const float* centre, const hr_light_shape* sh, const float* pos, uint32_t s, uint32_t S,
+                    float r1, float r2, float* out) {
+    v3 d = sampleRect(ld3(centre), *sh, ld3(pos), s, S, r1, r2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void or_glass_raygen(const float* N, const float* V, float ior, float* out) {
+    v3 d = glass_raygen_dir(ld3(N), ld3(V), ior);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+void or_glass_hit(const float* N, const float* rd, float ior, float* out) {
+    v3 d = glass_hit_dir(ld3(N), ld3(rd), ior);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
 }
 int or_intersect_tri(const float* org, const float* dir, float tmin, float tmax, const float* v0,
                      const float* v1, const float* v2, float* t, float* u, float* v) {
@@ -1662,16 +1723,7 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
             v3 origin = position3;
             v3 direction = mk(0, 0, 0);
             if (mode == 3) direction = reflect(normalize(neg(V)), N);
-            if (mode == 4) {
-                float ior = mat.diffuse_ior[3];
-                float cosAlpha = dot(N, V);
-                v3 I = neg(V);
-                float NdotI = dot(N, I);
-                v3 refrNormal = NdotI > 0.0f ? neg(N) : N;
-                float refrEta = NdotI > 0.0f ? 1.0f / ior : ior;
-                float radicand = 1.0f + refrEta * refrEta * (cosAlpha * cosAlpha - 1.0f);
-                direction = radicand < 0.0f ? reflect(I, N) : refract(I, refrNormal, refrEta);
-            }
+            if (mode == 4) direction = glass_raygen_dir(N, V, mat.diffuse_ior[3]);
             for (uint32_t i = 0; i < nl; ++i) {
                 const hr_light& lt = lights[i];
                 bool isDir = lt.pos_maxdist[3] < 0.0f;
@@ -1698,7 +1750,7 @@ void or_shade_hybrid(const or_scene* s, uint32_t W, uint32_t H, const float* pos
                     v3 Hh = normalize(add(V, L));
                     float D = DistributionGGX(N, Hh, roughness);
                     float G = GeometrySmith(N, V, L, roughness);
-                    v3 F = FresnelSchlick(fmaxf(dot(Hh, V), 0.0f), F0);
+                    v3 F = FresnelSchlick(fmaxf(g_plant == PLANT_FRESNEL_NDOTH ? dot(N, Hh) : dot(Hh, V), 0.0f), F0);
                     v3 kD = scl(sub(mk(1, 1, 1), F), 1.0f - metallic);
                     v3 numerator = scl(F, D * G);
                     float denominator = 4.0f * NdotV * NdotL;

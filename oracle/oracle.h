@@ -46,6 +46,19 @@ float or_geometry_smith(const float* N, const float* V, const float* L, float ro
 void or_fresnel_schlick(float cos_theta, const float* F0, float* out3);
 void or_sample_disk(const hr_light* light, const float* pos, const float* L,
                     float r1, float r2, float* dir_out);
+/* sampling and glass hooks (for closed-form and distribution checks):
+ * sampleGGX half-vector around unit N; sampleRect direction of stratum s of S
+ * from pos towards the rectangle light centred at centre; the glass direction
+ * of hybridRaygen.rgen:131-144 (V towards the eye) and of hybridHit.rchit:
+ * 158-173 (rd = incoming ray direction). */
+void or_sample_ggx(const float* N, float roughness, float r1, float r2, float* out3);
+void or_sample_rect(const float* centre, const hr_light_shape* shape, const float* pos, uint32_t s, uint32_t S,
+                    float r1, float r2, float* out3);
+void or_glass_raygen(const float* N, const float* V, float ior, float* out3);
+void or_glass_hit(const float* N, const float* rd, float ior, float* out3);
+/* TESTS ONLY: plant one deliberate mistake (see PLANT_* in hr_oracle.cpp;
+ * 0 restores the faithful oracle) to show that a check catches it. */
+void or_set_plant(int plant);
 /* Watertight ray/triangle test: returns 1 on a hit with t in [tmin, tmax]. */
 int or_intersect_tri(const float* org, const float* dir, float tmin, float tmax,
                      const float* v0, const float* v1, const float* v2,

@@ -87,6 +87,11 @@ def lib():
         L.or_geometry_smith.argtypes = [vp, vp, vp, f32]
         L.or_fresnel_schlick.argtypes = [f32, vp, vp]
         L.or_sample_disk.argtypes = [C.POINTER(abi.hr_light), vp, vp, f32, f32, vp]
+        L.or_sample_ggx.argtypes = [vp, f32, f32, f32, vp]
+        L.or_sample_rect.argtypes = [vp, C.POINTER(abi.hr_light_shape), vp, u32, u32, f32, f32, vp]
+        L.or_glass_raygen.argtypes = [vp, vp, f32, vp]
+        L.or_glass_hit.argtypes = [vp, vp, f32, vp]
+        L.or_set_plant.argtypes = [i32]
         L.or_intersect_tri.restype = i32
         L.or_intersect_tri.argtypes = [vp, vp, f32, f32, vp, vp, vp, vp, vp, vp]
         L.or_scene_build.restype = i32
@@ -339,6 +344,57 @@ def sample_disk(light_row, pos, L, r1, r2):
     out = np.zeros(3, dtype=np.float32)
     lib().or_sample_disk(la, pos.ctypes.data, L.ctypes.data, r1, r2, out.ctypes.data)
     return out
+
+
+def sample_ggx(N, roughness, r1, r2):
+    N = _v(N)
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_sample_ggx(N.ctypes.data, roughness, r1, r2, out.ctypes.data)
+    return out
+
+
+def sample_rect(centre, edge_u, edge_v, pos, s, S, r1, r2):
+    sh = abi.hr_light_shape()
+    sh.type = 1
+    sh.edge_u[:] = [float(x) for x in edge_u]
+    sh.edge_v[:] = [float(x) for x in edge_v]
+    c, p = _v(centre), _v(pos)
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_sample_rect(c.ctypes.data, C.byref(sh), p.ctypes.data, s, S, r1, r2, out.ctypes.data)
+    return out
+
+
+def glass_raygen(N, V, ior):
+    """direction of hybridRaygen.rgen:131-144 (V: unit vector towards the eye)"""
+    N, V = _v(N), _v(V)
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_glass_raygen(N.ctypes.data, V.ctypes.data, ior, out.ctypes.data)
+    return out
+
+
+def glass_hit(N, rd, ior):
+    """direction of hybridHit.rchit:158-173 (rd: the incoming ray direction)"""
+    N, rd = _v(N), _v(rd)
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_glass_hit(N.ctypes.data, rd.ctypes.data, ior, out.ctypes.data)
+    return out
+
+
+class planted:
+    """Context manager (tests only): the oracle with one deliberate mistake
+    (PLANT_* in hr_oracle.cpp) for the duration of the block."""
+    GLASS_ETA, GLASS_NORMAL, FRESNEL_NDOTH, GGX_ALPHA, RECT_STRATA, TIR_REFRACT = 1, 2, 3, 4, 5, 6
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().or_set_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_plant(0)
+        return False
 
 
 def intersect_tri(org, d, tmin, tmax, v0, v1, v2):
