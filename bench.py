@@ -8,14 +8,26 @@ lights with 16 stratified shadow samples each, GGX glossy reflections at
 4 spp, 1 bounce). value = traced rays (shadow + reflection + hit shadow) per
 second of the K timed steps; primary rays (the G-buffer) are excluded.
 
-roofline: the shadow kernel (most of the rays) against HBM, with algorithmic
-bytes = nodes visited x 64 B + triangles tested x 48 B + G-buffer position and
-normal read (32 B/px) + shadow factors written (4 B/px/light), from a
-counting launch outside the timed region, over its average launch time
-measured with HIP events on the library's stream.
+roofline: the shadow kernel (most of the rays) against the unit that binds
+it, the vector-memory address path: its wave-level vector load instructions
+per launch (rocprofv3 SQ_INSTS_VMEM_RD, from the committed profile of the same
+workload) over its average launch time measured live with HIP events on the
+library's stream, against 256 CUs x 2.4 GHz / 16 cycles (a CU's texture
+addresser takes 4 lanes per clock: 16 clocks per 64-lane load). Beside it: the
+measured TA busy fraction and VALU issue (a wave64 instruction per SIMD every
+4 cycles) with the lane utilisation, the algorithmic HBM bytes of the traced
+rays (32 B per ray + 4 B per shadow hit + 32 B per binary node visited + 48 B
+per triangle tested, the counts from a counting launch that walks the oracle's
+order, outside the timed region) and the HBM bytes rocprofv3 measured, both
+against 8 TB/s.
 
 cpu_baseline: the CPU oracle (test infrastructure; a scalar C++ port of the
-same pass) timed on a row subset of the same frame on the host's cores.
+same pass) timed on the same frame (a row subset for the larger workloads) on
+the process's share of host threads, with the CPU model named.
+
+parity: the same frame's shadow factors and lit / reflection colours against
+the oracle's on those rows (bit-exact shadows, radiance within 1e-4), and the
+ray counts of the timed pass against the oracle's.
 """
 import argparse
 import json
@@ -29,6 +41,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# wave-level VALU issue peak of an MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, one
+# wave64 VALU instruction per SIMD every 4 cycles
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+# wave-level vector memory instruction peak: a CU's texture addresser takes the
+# addresses of 4 lanes per clock, 16 clocks per 64-lane instruction
+VMEM_PEAK_GINST = 256 * 2.4 / 16
+RAD_TOL = 1e-4
 
 
 def log(*a):
@@ -57,17 +76,71 @@ def shadow_kernel_name(params):
     return "k_shadows_spl"
 
 
-def load_traffic(workload):
-    """HBM bytes per launch of the shadow kernel measured by rocprofv3 --pmc
-    runs and committed under profiles/, with the L2 hit rate, lane
-    utilisation, VALU busy and occupancy of the same runs (None when absent)."""
-    p = os.path.join(ROOT, "profiles", "k_shadows_%s_traffic.json" % workload)
+def load_profile(kernel, workload):
+    """The per-launch counters of `kernel` on `workload` that rocprofv3 runs
+    measured (profiles/<kernel>_<workload>.json): VALU instructions, lane
+    utilisation, TA busy, HBM bytes, L2 hit rate (None when absent)."""
+    p = os.path.join(ROOT, "profiles", "%s_%s.json" % (kernel, workload))
     if not os.path.exists(p):
-        return None, None, {}
-    d = json.load(open(p))
-    counters = {k: d[k] for k in ("l2_hit_rate", "valu_utilization", "valu_busy", "occupancy_waves_per_simd",
-                                  "achieved_hbm_gbs") if k in d}
-    return d.get("hbm_bytes_per_launch"), os.path.relpath(p, ROOT), counters
+        return None, None
+    return json.load(open(p)), os.path.relpath(p, ROOT)
+
+
+def roofline_of(kernel, workload, ms, rays, nodes, tris):
+    """Roofline of one launch of a traversal kernel that ran `ms` on average:
+    VALU issue (binding) with the HBM figures beside it."""
+    prof, src = load_profile(kernel, workload)
+    alg = 36 * rays + 32 * nodes + 48 * tris
+    alg_gbs = alg / (ms * 1e-3) / 1e9
+    out = {"kernel": kernel, "avg_launch_ms": round(ms, 3),
+           "hbm_algorithmic": {"bytes": int(alg), "achieved": round(alg_gbs, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(alg_gbs / HBM_PEAK_GBS, 4),
+                               "per_ray": "32 B ray + 4 B hit + 32 B per binary node + 48 B per triangle",
+                               "nodes_per_ray": round(nodes / max(rays, 1), 2),
+                               "tris_per_ray": round(tris / max(rays, 1), 2)}}
+    if prof is None:
+        out.update(bound="hbm", achieved=out["hbm_algorithmic"]["achieved"], peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=out["hbm_algorithmic"]["frac"], traffic=None)
+        return out
+    vm = prof["vmem_read_insts_per_launch"]
+    ach = vm / (ms * 1e-3) / 1e9
+    traffic = prof.get("hbm_bytes_per_launch")
+    vi = prof["valu_insts_per_launch"]
+    valu = vi / (ms * 1e-3) / 1e9
+    out.update(bound="vmem", achieved=round(ach, 2), peak=VMEM_PEAK_GINST, unit="Ginst/s",
+               frac=round(ach / VMEM_PEAK_GINST, 4), traffic=traffic, profile=src,
+               vmem_insts_per_launch=vm, ta_busy=prof.get("ta_busy"), l2_hit_rate=prof.get("l2_hit_rate"),
+               valu={"insts_per_launch": vi, "achieved": round(valu, 1), "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+                     "issue_frac": round(valu / VALU_PEAK_GINST, 4),
+                     "lane_utilization": prof.get("lane_utilization"),
+                     "useful_lane_frac": round(valu / VALU_PEAK_GINST * prof.get("lane_utilization", 0.0), 4)})
+    if traffic:
+        gbs = traffic / (ms * 1e-3) / 1e9
+        out["hbm_measured"] = {"bytes": traffic, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    return out
+
+
+def frame_parity(sh, osh, lit, olit, refl, orefl):
+    """mismatch figures of GPU buffers against the oracle's on the same pixels"""
+    sbad = ~((sh == osh) | (np.isnan(sh) & np.isnan(osh)))
+    nan_same = bool(np.array_equal(np.isnan(lit), np.isnan(olit)))
+    dl = np.nan_to_num(np.abs(lit[..., :3] - olit[..., :3]))
+    dr = np.nan_to_num(np.abs(refl[..., :3] - orefl[..., :3]))
+    return {"shadow_mismatch_frac": float(sbad.mean()), "pixels": int(lit.shape[0] * lit.shape[1]),
+            "lit_mismatch_frac": float((dl > RAD_TOL).any(-1).mean()), "lit_max_abs": float(dl.max()),
+            "refl_mismatch_frac": float((dr > RAD_TOL).any(-1).mean()), "refl_max_abs": float(dr.max()),
+            "nan_pattern_equal": nan_same, "tolerance": "shadow bit-exact, radiance %g absolute" % RAD_TOL}
+
+
+def cpu_info(threads):
+    from oracle import pyoracle
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"cores": threads, "cpu_model": pyoracle.cpu_model(), "host_threads": os.cpu_count(),
+            "affinity_threads": aff, "thread_share": pyoracle.HOST_THREAD_SHARE}
 
 
 def batch_main(args):
@@ -227,7 +300,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="hybrid_1080p_1m")
-    ap.add_argument("--cpu-rows-step", type=int, default=3)
+    ap.add_argument("--cpu-rows-step", type=int, default=0,
+                    help="oracle rows y = 0, k, 2k ... (0: every row for the 1M-triangle frame, every 4th otherwise)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--leaf-max", type=int, default=2, help="largest subtree stored as one BVH leaf")
     ap.add_argument("--wave-budget-mb", type=int, default=0,
@@ -307,36 +381,39 @@ def main():
                     hit_shadow_rays=int(st_h.hit_shadow_rays), occluded_rays=int(st_s.occluded_rays +
                                                                                   st_h.occluded_rays))
     W, H, nl = sc.width, sc.height, len(sc.lights)
-    bytes_shadow = shadow_nodes * 64 + shadow_tris * 48 + W * H * (32 + 4 * nl)
     avg_shadow_ms = float(np.mean(ms_shadow))
-    achieved = bytes_shadow / (avg_shadow_ms * 1e-3) / 1e9
-    traffic, traffic_src, counters = load_traffic(sc.name)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": shadow_kernel_name(sc.params),
-                "algorithmic_bytes": int(bytes_shadow),
-                "avg_launch_ms": round(avg_shadow_ms, 3),
-                "nodes_per_ray": round(shadow_nodes / max(shadow_rays, 1), 2),
-                "tris_per_ray": round(shadow_tris / max(shadow_rays, 1), 2)}
-    if counters:
-        roofline["rocprof"] = counters  # measured HBM GB/s, L2 hit rate, VALU, occupancy (profiles/)
+    kname = shadow_kernel_name(sc.params)
+    roofline = roofline_of(kname.split()[0], sc.name, avg_shadow_ms, shadow_rays, shadow_nodes, shadow_tris)
+    if kname != roofline["kernel"]:
+        roofline["kernels"] = kname
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
         from oracle import pyoracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or pyoracle.default_threads()
+        step = args.cpu_rows_step or (1 if sc.name == "hybrid_1080p_1m" else 4)
         gb = rd.gb.download()
-        sh = rd.download_shadow()
+        sh, lit, refl = rd.download_shadow(), rd.download_lit(), rd.download_refl()
         os_ = pyoracle.OracleScene(sc, leaf_max=args.leaf_max, threads=threads)
-        rows = (0, args.cpu_rows_step)
+        rows = (0, step)
         t1 = time.perf_counter()
-        _, c1 = os_.trace_shadows(gb, rows=rows)
-        _, _, c2 = os_.shade_hybrid(gb, sh, rows=rows)
+        osh, c1 = os_.trace_shadows(gb, rows=rows)
+        olit, orefl, c2 = os_.shade_hybrid(gb, sh, rows=rows)
         tc = time.perf_counter() - t1
         crays = c1["shadow_rays"] + c2["reflection_rays"] + c2["hit_shadow_rays"]
-        cpu = {"value": round(crays / tc / 1e9, 6), "unit": "Grays/s", "cores": threads, "kind": "port",
-               "sample": "rows 0::%d of the same frame (%d rays, %.1f s)" % (args.cpu_rows_step, crays, tc)}
+        cpu = {"value": round(crays / tc / 1e9, 6), "unit": "Grays/s", **cpu_info(threads), "kind": "port",
+               "sample": "%s of the timed frame (%d rays, %.1f s)" % (
+                   "every row" if step == 1 else "rows 0::%d" % step, crays, tc)}
         log("cpu baseline", cpu)
+        sel = slice(0, None, step)
+        parity = frame_parity(sh[:, sel], osh[:, sel], lit[sel], olit[sel], refl[sel], orefl[sel])
+        if step == 1:  # the whole frame: the timed pass's ray counts are the oracle's
+            parity["counts_equal"] = bool(per_step["shadow_rays"] == c1["shadow_rays"] and
+                                          per_step["reflection_rays"] == c2["reflection_rays"] and
+                                          per_step["hit_shadow_rays"] == c2["hit_shadow_rays"] and
+                                          per_step["occluded_rays"] == c1["occluded_rays"] + c2["occluded_rays"])
+        parity["rows"] = "all" if step == 1 else "0::%d" % step
+        log("parity", parity)
 
     metric = "Grays/s (1080p, 1M tris)" if sc.name == "hybrid_1080p_1m" else "Grays/s (%s)" % sc.name
     out = {"metric": metric, "value": round(value, 4), "unit": "Grays/s", "n_gpus": 1,
@@ -348,7 +425,21 @@ def main():
                       "max_depth": sc.params["max_depth"], "leaf_max": args.leaf_max},
            "rays_per_step": per_step, "ms_shadow": round(avg_shadow_ms, 3),
            "ms_shade": round(float(np.mean(ms_shade)), 3), "ms_gbuffer": round(gst["ms_gbuffer"], 3),
-           "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu}
+           "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu,
+           "parity": parity}
+    # the closest-hit queue kernel of the reflection stage, from its own profile
+    # (it runs inside hr_shade_hybrid, so its launch time is rocprofv3's)
+    prof_r, src_r = load_profile("k_wf_trace", sc.name)
+    if prof_r:
+        ms_r = prof_r["avg_launch_ms_traced"]
+        ach = prof_r["vmem_read_insts_per_launch"] / (ms_r * 1e-3) / 1e9
+        valu = prof_r["valu_insts_per_launch"] / (ms_r * 1e-3) / 1e9
+        out["roofline_closest_hit"] = {"kernel": "k_wf_trace", "bound": "vmem", "achieved": round(ach, 2),
+                                       "peak": VMEM_PEAK_GINST, "unit": "Ginst/s",
+                                       "frac": round(ach / VMEM_PEAK_GINST, 4), "avg_launch_ms_rocprof": ms_r,
+                                       "ta_busy": prof_r.get("ta_busy"),
+                                       "valu_issue_frac": round(valu / VALU_PEAK_GINST, 4),
+                                       "lane_utilization": prof_r.get("lane_utilization"), "profile": src_r}
     if sc.name == "hybrid_1080p_1m" and not args.no_denoise:
         del rd, hs
         ctx.close()
