@@ -238,6 +238,17 @@ typedef struct hr_stats {
  *  - reflections: a glossy pixel (shading model 3, roughness r > 0) traces
  *    1 + floor((refl_spp - 1) * r) paths. */
 #define HR_FLAG_ADAPTIVE 4u
+/* The reference's 8-bit images (no other change to the results):
+ *  HR_FLAG_SHADOW_R8: the shadow buffer is the R8_UNORM image of
+ *    renderer.cpp:1538 (uint8[nl][H][W]): hr_trace_shadows stores
+ *    rint(clamp(factor, 0, 1) * 255) and hr_shade_hybrid reads code / 255, as
+ *    imageStore / imageLoad do (shadowRaygen.rgen:93, hybridRaygen.rgen:83);
+ *  HR_FLAG_LIT_BGRA8: hr_shade_hybrid stores lit as the B8G8R8A8_UNORM image of
+ *    renderer.cpp:1535 (uint8[H][W][4], blue first, alpha 255):
+ *    rint(clamp(colour, 0, 1) * 255) before the post gamma (hybridRaygen.rgen:
+ *    186). refl stays FP32. */
+#define HR_FLAG_SHADOW_R8 8u
+#define HR_FLAG_LIT_BGRA8 16u
 
 typedef struct hr_build_params {
     uint32_t leaf_max;   /* largest Karras subtree stored as one leaf (1..8, default 2) */

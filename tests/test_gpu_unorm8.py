@@ -1,0 +1,144 @@
+"""GPU: the reference's 8-bit images (renderer.cpp:1535-1538).
+
+- HR_FLAG_SHADOW_R8: the shadow pass stores rint(clamp(f, 0, 1) * 255) of the
+  factor it computes in FP32 (imageStore to R8_UNORM, shadowRaygen.rgen:93):
+  bit-exact against the quantised FP32 factors, which equal the oracle's; on
+  every shadow kernel path (sample-parallel, adaptive probe / refine, group
+  per lane, batches);
+- the hybrid pass then reads code / 255 (imageLoad, hybridRaygen.rgen:83): lit
+  within 1e-4 of the oracle shading from the same quantised factors;
+- HR_FLAG_LIT_BGRA8: lit stored as B8G8R8A8_UNORM (hybridRaygen.rgen:186 into
+  the image of renderer.cpp:1535): bytes (B, G, R, 255) = the quantised FP32
+  colour of the same pass, bit-exact.
+"""
+import numpy as np
+import pytest
+
+from vulkanhybridraytracer_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+R8, BGRA8 = abi.HR_FLAG_SHADOW_R8, abi.HR_FLAG_LIT_BGRA8
+
+
+def q8(v):
+    return np.rint(np.clip(np.nan_to_num(v, nan=0.0), 0.0, 1.0) * np.float32(255.0)).astype(np.uint8)
+
+
+def _renderer(gpu_ctx, sc):
+    from vulkanhybridraytracer_amd import hr
+    rd = hr.HybridRenderer(gpu_ctx, hr.HybridScene(gpu_ctx, sc))
+    rd.gbuffer_primary()
+    return rd
+
+
+def _shadow_bytes(rd):
+    return rd.shadow.download((rd.nl, rd.height, rd.width), np.uint8)
+
+
+@pytest.mark.parametrize("samples,adaptive", [(16, False), (32, True), (5, False), (1, False)])
+def test_shadow_r8_is_the_quantised_factor(gpu_ctx, oracle_lib, samples, adaptive):
+    sc = scenes.make_terrain_scene(seed=3, width=240, height=136, quads=96, instances=24, soup_tris=256,
+                                   samples=samples, spp=4, adaptive=adaptive)
+    rd = _renderer(gpu_ctx, sc)
+    flags = rd.params.flags
+    rd.trace_shadows()
+    f = rd.download_shadow()
+    rd.params.flags = flags | R8
+    rd.shadow.zero()
+    rd.trace_shadows()
+    q = _shadow_bytes(rd)
+    assert np.array_equal(q, q8(f))
+    # and the oracle's factors on every 4th row quantise to the same codes
+    gb = rd.gb.download()
+    osh, _ = oracle_lib.OracleScene(sc).trace_shadows(gb, rows=(1, 4))
+    assert np.array_equal(q[:, 1::4], q8(osh[:, 1::4]))
+    assert ((f > 0) & (f < 1)).any() or samples == 1
+
+
+def test_shading_reads_r8_shadows(gpu_ctx, oracle_lib):
+    sc = scenes.make_terrain_scene(seed=4, width=240, height=136, quads=96, instances=24, soup_tris=256,
+                                   samples=16, spp=4)
+    rd = _renderer(gpu_ctx, sc)
+    rd.params.flags |= R8
+    rd.trace_shadows()
+    rd.shade_hybrid()
+    q = _shadow_bytes(rd)
+    lit = rd.download_lit()
+    gb = rd.gb.download()
+    sh = (q.astype(np.float32) / np.float32(255.0)).astype(np.float32)
+    os_ = oracle_lib.OracleScene(sc)
+    olit, _, _ = os_.shade_hybrid(gb, sh, rows=(2, 4))
+    d = np.nan_to_num(np.abs(lit[2::4] - olit[2::4]))
+    assert ((d > 1e-4).any(-1)).mean() < 1e-5, d.max()
+    # the quantisation reaches the colour: shading from the FP32 factors differs
+    rd.params.flags &= ~R8
+    rd.trace_shadows()
+    rd.shade_hybrid()
+    assert not np.array_equal(rd.download_lit(), lit)
+
+
+def test_lit_bgra8_is_the_quantised_colour(gpu_ctx):
+    sc = scenes.make_terrain_scene(seed=5, width=240, height=136, quads=96, instances=24, soup_tris=256,
+                                   samples=8, spp=4)
+    rd = _renderer(gpu_ctx, sc)
+    rd.trace_shadows()
+    rd.shade_hybrid()
+    lit = rd.download_lit()
+    rd.params.flags |= BGRA8
+    rd.lit.zero()
+    rd.shade_hybrid()
+    b = rd.lit.download((rd.height, rd.width, 4), np.uint8)
+    want = np.stack([q8(lit[..., 2]), q8(lit[..., 1]), q8(lit[..., 0]), np.full(lit.shape[:2], 255, np.uint8)], -1)
+    assert np.array_equal(b, want)
+    assert (b[..., :3] > 0).any() and (b[..., :3] < 255).any()
+
+
+def test_batch_with_8bit_frames(gpu_ctx):
+    """a batch mixing FP32 and 8-bit frames gives each frame what it gives alone"""
+    from vulkanhybridraytracer_amd import hr
+    batch = scenes.make_batch(count=3, tri_range=(8_000, 30_000), resolutions=((96, 64), (64, 48)), spp=8)
+    rds = [_renderer(gpu_ctx, sc) for sc in batch]
+    rds[0].params.flags |= R8 | BGRA8
+    rds[2].params.flags |= R8
+
+    def grab(rd):
+        """the bytes the frame's formats define (1 or 4 per factor, 4 or 16 per pixel)"""
+        ns = rd.nl * rd.height * rd.width * (1 if rd.params.flags & R8 else 4)
+        nlit = rd.height * rd.width * (4 if rd.params.flags & BGRA8 else 16)
+        return rd.shadow.download((ns,), np.uint8), rd.lit.download((nlit,), np.uint8)
+
+    def clear():
+        for rd in rds:
+            rd.shadow.zero()
+            rd.lit.zero()
+
+    clear()
+    ref = []
+    for rd in rds:
+        rd.trace_shadows()
+        rd.shade_hybrid()
+        ref.append(grab(rd))
+    clear()
+    br = hr.BatchRenderer(gpu_ctx, rds)
+    g = br.capture()
+    for how in ("graph", "eager", "graph"):
+        if how == "graph":
+            g.launch()
+        else:
+            br.trace_shadows()
+            br.shade()
+        for k, ((s, l), rd) in enumerate(zip(ref, rds)):
+            gs, gl = grab(rd)
+            assert np.array_equal(gs, s), (how, k, int((gs != s).sum()), int((s != 0).sum()), int((gs != 0).sum()))
+            assert np.array_equal(gl, l), (how, k, int((gl != l).sum()))
+        clear()
+    g.close()
+
+
+def test_unknown_trace_param_flags_are_rejected(gpu_ctx):
+    from vulkanhybridraytracer_amd import hr
+    rd = _renderer(gpu_ctx, scenes.make_small(width=32, height=18, spheres=2))
+    rd.params.flags = 64
+    with pytest.raises(hr.HRError) as e:
+        rd.trace_shadows()
+    assert e.value.status == abi.HR_E_INVALID

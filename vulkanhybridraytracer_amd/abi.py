@@ -17,6 +17,7 @@ STATUS_NAMES = {0: "HR_OK", 1: "HR_E_INVALID", 2: "HR_E_OOM", 3: "HR_E_DEVICE",
 HR_MAX_LIGHTS = 16
 HR_LIGHT_DISK, HR_LIGHT_RECT = 0, 1
 HR_FLAG_COUNT_VISITS, HR_FLAG_TIMING, HR_FLAG_ADAPTIVE = 1, 2, 4
+HR_FLAG_SHADOW_R8, HR_FLAG_LIT_BGRA8 = 8, 16  # the reference's R8_UNORM shadow / B8G8R8A8_UNORM lit images
 HR_TRACE_ANY_HIT, HR_TRACE_COUNT_VISITS = 1, 2  # hr_trace_rays flags
 
 

@@ -191,6 +191,10 @@ hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, con
     f.spp = prm->refl_spp;
     f.max_depth = prm->max_depth;
     f.adaptive = (prm->flags & HR_FLAG_ADAPTIVE) ? 1u : 0u;
+    const uint32_t known = HR_FLAG_COUNT_VISITS | HR_FLAG_TIMING | HR_FLAG_ADAPTIVE | HR_FLAG_SHADOW_R8 |
+                           HR_FLAG_LIT_BGRA8;
+    if (prm->flags & ~known) return fail(ctx, HR_E_INVALID, "unknown trace flags 0x%x", prm->flags & ~known);
+    f.unorm8 = ((prm->flags & HR_FLAG_SHADOW_R8) ? UNORM8_SHADOW : 0u) | ((prm->flags & HR_FLAG_LIT_BGRA8) ? UNORM8_LIT : 0u);
     return HR_OK;
 }
 hr_status check_gbuffer(hr_context* ctx, const hr_gbuffer* gb) {

@@ -109,7 +109,10 @@ struct FrameArgs {
     uint32_t samples, spp, max_depth;
     uint32_t has_shapes;
     uint32_t adaptive;  // HR_FLAG_ADAPTIVE
+    uint32_t unorm8;    // UNORM8_SHADOW | UNORM8_LIT: the reference's 8-bit image formats
 };
+constexpr uint32_t UNORM8_SHADOW = 1u;  // shadow buffers are R8_UNORM
+constexpr uint32_t UNORM8_LIT = 2u;     // lit is B8G8R8A8_UNORM
 
 struct GBufView {
     float4* pos;

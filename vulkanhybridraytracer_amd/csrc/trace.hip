@@ -199,7 +199,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     const TNode* nodes = jobs[0].S.nodes;
     const STri* tris = jobs[0].S.tris;
     uint32_t node_count = jobs[0].S.node_count;
-    float* out = nullptr;
+    float* out_base = nullptr;  // the group's factor: out_base[out_idx], FP32 or R8 by out_fmt
+    size_t out_idx = 0;
+    uint32_t out_fmt = 0;
     uint32_t li = 0, seed = 0, s = 0, unocc = 0, SS = 1;
     uint32_t target = 1;    // rays of the current phase
     bool refine = false;    // adaptive: the penumbra phase after disagreeing probes
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                         s = 0;
                         unocc = 0;
                     } else {
-                        *out = (float)unocc / (float)target;
+                        put_factor(out_base, out_idx, (float)unocc / (float)target, out_fmt);
                         in_group = false;
                     }
                 }
@@ -292,12 +294,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     tile_pixel(J, t, q, x, y);
                     if (x >= F.width || y >= F.height) continue;
                     size_t p = (size_t)y * F.width + x;
-                    out = J.shadow + li * ((size_t)F.width * F.height) + p;
+                    out_base = J.shadow;
+                    out_idx = li * ((size_t)F.width * F.height) + p;
+                    out_fmt = F.unorm8;
                     float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
                     bool background = P4.w == 0.0f && N4.w == 0.0f;
                     int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
                     if (background || mode == 4) {
-                        *out = background ? 0.0f : 1.0f;
+                        put_factor(out_base, out_idx, background ? 0.0f : 1.0f, out_fmt);
                         continue;
                     }
                     P = ld3(P4);
@@ -309,11 +313,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     L = normalize(L);
                     float NdotL = clamp01(dot(N, L));
                     if (!(NdotL > 0.0f)) {
-                        *out = 0.0f;
+                        put_factor(out_base, out_idx, 0.0f, out_fmt);
                         continue;
                     }
                     if (!(isDir || dist < lt.pos_maxdist[3])) {
-                        *out = 1.0f;
+                        put_factor(out_base, out_idx, 1.0f, out_fmt);
                         continue;
                     }
                     seed = tea(y * F.width + x, F.frame);
@@ -646,7 +650,8 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
     tile_pixel(J, t, q, x, y);
     if (x >= F.width || y >= F.height) return;
     const uint32_t n = window_bits(bits, (uint64_t)G << lg2s, lg2s);
-    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
+    put_factor(J.shadow, li * ((size_t)F.width * F.height) + (size_t)y * F.width + x, (float)n / (float)F.samples,
+               F.unorm8);
 }
 
 // Adaptive sampling, after the probes: a hard light's factor is its one
@@ -668,9 +673,11 @@ __global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restri
     tile_pixel(J, t, q, x, y);
     if (x >= F.width || y >= F.height) return;
     const uint32_t w = (bits[G >> 3] >> ((G & 7u) * 4u)) & 15u;  // the 4 probe bits of the group
-    float* out = J.shadow + li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
+    float* out_base = J.shadow;
+    const size_t out_idx = li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
+    const uint32_t out_fmt = F.unorm8;
     if (hard_light(F, li)) {
-        *out = (float)(w & 1u) / 1.0f;
+        put_factor(out_base, out_idx, (float)(w & 1u) / 1.0f, out_fmt);
         return;
     }
     const uint32_t n = (uint32_t)__popc(w);
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restri
         list[atomicAdd(list_count, 1u)] = G;
         return;
     }
-    *out = (float)n / 4.0f;
+    put_factor(out_base, out_idx, (float)n / 4.0f, out_fmt);
 }
 
 // factors of the refined groups
@@ -698,7 +705,8 @@ __global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __r
     uint32_t x, y;
     tile_pixel(J, t, q, x, y);
     const uint32_t n = window_bits(bits, (uint64_t)slot << lg2s, lg2s);
-    J.shadow[li * ((size_t)F.width * F.height) + (size_t)y * F.width + x] = (float)n / (float)F.samples;
+    put_factor(J.shadow, li * ((size_t)F.width * F.height) + (size_t)y * F.width + x, (float)n / (float)F.samples,
+               F.unorm8);
 }
 
 }  // namespace

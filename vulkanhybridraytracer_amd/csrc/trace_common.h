@@ -472,6 +472,25 @@ __device__ __forceinline__ bool hard_light(const FrameArgs& F, uint32_t i) {
     return !rect && lt.radius == 0.0f;
 }
 
+// ---- the reference's 8-bit images --------------------------------------------
+// imageStore to a UNORM8 channel: rint(clamp(v, 0, 1) * 255) (NaN stores 0)
+__device__ __forceinline__ uint32_t unorm8(float v) { return (uint32_t)rintf(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f); }
+// shadow factor idx of an FP32 or R8_UNORM shadow buffer
+__device__ __forceinline__ void put_factor(float* shadow, size_t idx, float v, uint32_t fmt) {
+    if (fmt & UNORM8_SHADOW) reinterpret_cast<uint8_t*>(shadow)[idx] = (uint8_t)unorm8(v);
+    else shadow[idx] = v;
+}
+__device__ __forceinline__ float get_factor(const float* shadow, size_t idx, uint32_t fmt) {
+    if (fmt & UNORM8_SHADOW) return (float)reinterpret_cast<const uint8_t*>(shadow)[idx] / 255.0f;
+    return shadow[idx];
+}
+// lit colour of pixel p (alpha 1) to an FP32 RGBA or a B8G8R8A8_UNORM image
+__device__ __forceinline__ void put_lit(float4* lit, size_t p, v3 c, uint32_t fmt) {
+    if (fmt & UNORM8_LIT)
+        reinterpret_cast<uint32_t*>(lit)[p] = unorm8(c.z) | (unorm8(c.y) << 8) | (unorm8(c.x) << 16) | (255u << 24);
+    else lit[p] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
 __device__ __forceinline__ int mat_index(const DevScene& S, float w) {
     int i = (int)w;
     return (i < 0 || i >= (int)S.nmat) ? 0 : i;

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
         v3 emis = ld3(G.emi[p]);
         bool background = P4.w == 0.0f && N4.w == 0.0f;
         if (background) {
-            lit[p] = make_float4(albedo.x, albedo.y, albedo.z, 1.0f);
+            put_lit(lit, p, albedo, F.unorm8);
             refl[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
             W.pix[2 * (size_t)(p - W.p0) + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         } else {
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
                 float intensity = isDir ? 1.0f : (lt.color_intensity[3] / (dist * dist));
                 L = normalize(L);
                 float NdotL = clamp01(dot(N, L));
-                float shadowFactor = shadow[i * plane + p];
+                float shadowFactor = get_factor(shadow, i * plane + p, F.unorm8);
                 if (NdotL > 0.0f) {
                     if (intensity == 0.0f) {
                         attenuation = 0.0f;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_direct(DevScene S, FrameArgs F,
             bool traced = (mode == 3 || mode == 4) && F.max_depth > 0 && F.nl > 0;
             if (!traced) {
                 v3 fc = add(add(rayColor, ambient), emis);
-                lit[p] = make_float4(fc.x, fc.y, fc.z, 1.0f);
+                put_lit(lit, p, fc, F.unorm8);
                 refl[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
                 W.pix[2 * (size_t)(p - W.p0) + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
             } else {
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(WF_BLOCK) void k_wf_resolve(FrameArgs F, GBufView G
     v3 rfac = nspp == 1 ? acc : divs(acc, (float)nspp);
     v3 rayColor = mul(ld3(a), rfac);
     v3 fc = add(add(rayColor, ld3(b)), ld3(G.emi[p]));
-    lit[p] = make_float4(fc.x, fc.y, fc.z, 1.0f);
+    put_lit(lit, p, fc, F.unorm8);
     refl[p] = make_float4(rfac.x, rfac.y, rfac.z, 1.0f);
 }
 
