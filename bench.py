@@ -68,11 +68,11 @@ def copy_probe(nbytes=1 << 30, reps=3):
 
 def shadow_kernel_name(params):
     """The kernel(s) of the timed shadow pass: sample-parallel for 4..64
-    samples (a power of two), in two phases with adaptive sampling."""
+    samples (a power of two), in stages with adaptive sampling."""
     if params["shadow_samples"] not in (4, 8, 16, 32, 64):
         return "k_shadows"
     if params.get("flags", 0) & 4:
-        return "k_shadows_spl (probe and refine phases, with k_shadow_decide)"
+        return "k_shadows_spl (probe stage and later stages, with k_adaptive_decide)"
     return "k_shadows_spl"
 
 

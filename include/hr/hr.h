@@ -230,11 +230,15 @@ typedef struct hr_stats {
 #define HR_FLAG_COUNT_VISITS 1u  /* count node visits / triangle tests (slower kernel variant) */
 #define HR_FLAG_TIMING 2u        /* reserved */
 /* Adaptive sampling (no counterpart in the reference, whose sample count is a
- * uniform, renderer.cpp:656):
- *  - shadows: a disk light of radius 0 takes 1 ray; other lights take 4
- *    stratified probe rays and, when the probes disagree (a penumbra) and
- *    shadow_samples > 4, shadow_samples stratified rays (at most 64) that
- *    replace them;
+ * uniform, renderer.cpp:656, GUI range 1..64):
+ *  - shadows: a disk light of radius 0 takes 1 ray. Other lights draw from
+ *    one nested stratified sequence of S = shadow_samples (a power of two, at
+ *    most 64) cells on the light, in stages of 4, 16, 32, 64 samples (up to S),
+ *    each stage adding to the samples before it. After N samples of which k
+ *    are unoccluded the count stops when N = S or when the binomial variance
+ *    estimate of the visible fraction, k (N - k) / N^3, is at most that of S
+ *    samples at one half, 1 / (4 S): a lit or umbral pixel takes 4 rays, a
+ *    penumbra 16, 32 or 64 by how mixed its samples are. The factor is k / N;
  *  - reflections: a glossy pixel (shading model 3, roughness r > 0) traces
  *    1 + floor((refl_spp - 1) * r) paths. */
 #define HR_FLAG_ADAPTIVE 4u

@@ -221,7 +221,7 @@ enum : uint32_t {
     PURPOSE_REFLECT = 2,        /* index = spp sample (GGX lobe at the G-buffer surface) */
     PURPOSE_HIT_SHADOW = 3,     /* index = (sample * max_depth + depth) * nl + light */
     PURPOSE_HIT_REFLECT = 4,    /* index = sample * max_depth + depth */
-    PURPOSE_ADAPTIVE_SHADOW = 5, /* index = light * 128 + k (k < 4 probes, 64 + s refinement) */
+    PURPOSE_ADAPTIVE_SHADOW = 5, /* index = light * 128 + s (s: sample of the nested sequence, < 64) */
     PURPOSE_RT_PIXEL = 6,        /* pure ray tracing: index = pixel sample (sub-pixel offset) */
     PURPOSE_RT_SHADOW = 7,       /* pure ray tracing: index = (sample * max_depth + depth) * nl + light */
     PURPOSE_RT_REFLECT = 8       /* pure ray tracing: index = sample * max_depth + depth */
@@ -286,7 +286,9 @@ enum : int {
     PLANT_FRESNEL_NDOTH = 3,  /* hybridRaygen.rgen:111 Fresnel of NdotH instead of HdotV */
     PLANT_GGX_ALPHA = 4,      /* sampleGGX with a = roughness instead of roughness^2 */
     PLANT_RECT_STRATA = 5,    /* sampleRect rows divided by n instead of m */
-    PLANT_TIR_REFRACT = 6     /* total internal reflection answered by refract() */
+    PLANT_TIR_REFRACT = 6,    /* total internal reflection answered by refract() */
+    PLANT_ADAPTIVE_STOP = 7,  /* stopping rule against N^2 instead of N^3 */
+    PLANT_NESTED_ORDER = 8    /* nested cells without the bit reversal */
 };
 static int g_plant = PLANT_NONE;
 
@@ -938,20 +940,51 @@ struct Frame {
     uint32_t frame;
     bool adaptive;
 };
-/* adaptive sampling: cell s of the n x ceil(S/n) grid on the light */
-static inline v3 light_sample_dir_strat(const Frame& f, uint32_t i, v3 P, v3 L, Pcg& g, uint32_t s, uint32_t S) {
+/* Adaptive sampling: sample s of a light's nested stratified sequence of S (a
+ * power of two <= 64) cells. Column cx of nx = 2^ceil(lg S / 2) from the bits
+ * of s at even positions, reversed; row cy of ny = S / nx from the bits at odd
+ * positions, reversed: every prefix of 4^j samples holds one sample per cell
+ * of the 2^j x 2^j coarsening. */
+static inline void nested_cell(uint32_t s, uint32_t S, uint32_t& cx, uint32_t& nx, uint32_t& cy, uint32_t& ny) {
+    uint32_t lg = 0;
+    while ((2u << lg) <= S) ++lg;
+    const uint32_t a = (lg + 1u) / 2u, b = lg / 2u;
+    cx = 0;
+    cy = 0;
+    const bool rev = g_plant != PLANT_NESTED_ORDER;
+    for (uint32_t k = 0; k < a; ++k) cx |= ((s >> (2u * k)) & 1u) << (rev ? a - 1u - k : k);
+    for (uint32_t k = 0; k < b; ++k) cy |= ((s >> (2u * k + 1u)) & 1u) << (rev ? b - 1u - k : k);
+    nx = 1u << a;
+    ny = 1u << b;
+}
+static inline v3 light_sample_dir_nested(const Frame& f, uint32_t i, v3 P, v3 L, Pcg& g, uint32_t s, uint32_t S) {
     const hr_light& lt = f.lights[i];
     float r1 = u01(g);
     float r2 = u01(g);
+    uint32_t cx, nx, cy, ny;
+    nested_cell(s, S, cx, nx, cy, ny);
+    float a = ((float)cx + r1) / (float)nx;
+    float b = ((float)cy + r2) / (float)ny;
     bool isDir = lt.pos_maxdist[3] < 0.0f;
-    if (!isDir && f.shapes && f.shapes[i].type == HR_LIGHT_RECT)
-        return sampleRect(mk(lt.pos_maxdist[0], lt.pos_maxdist[1], lt.pos_maxdist[2]), f.shapes[i], P, s, S, r1, r2);
-    uint32_t n = 1;
-    while ((n + 1) * (n + 1) <= S) ++n;
-    uint32_t m = (S + n - 1) / n;
-    float a = ((float)(s % n) + r1) / (float)n;
-    float b = ((float)(s / n) + r2) / (float)m;
+    if (!isDir && f.shapes && f.shapes[i].type == HR_LIGHT_RECT) {
+        const hr_light_shape& sh = f.shapes[i];
+        v3 eu = mk(sh.edge_u[0], sh.edge_u[1], sh.edge_u[2]);
+        v3 ev = mk(sh.edge_v[0], sh.edge_v[1], sh.edge_v[2]);
+        v3 target = add(add(mk(lt.pos_maxdist[0], lt.pos_maxdist[1], lt.pos_maxdist[2]), scl(eu, a - 0.5f)),
+                        scl(ev, b - 0.5f));
+        return normalize(sub(target, P));
+    }
     return sampleDisk(lt, P, L, a, b);
+}
+/* Stages: 4 probes (all S when S < 4), then 16, then doubling up to S; a
+ * stage is taken while the estimate k / N of the visible fraction has a
+ * binomial variance estimate k (N - k) / N^3 above that of S samples at one
+ * half, 1 / (4 S) (in integers). */
+static inline uint32_t adaptive_first(uint32_t S) { return S < 4u ? S : 4u; }
+static inline uint32_t adaptive_next(uint32_t N, uint32_t S) { return std::min(N == 4u ? 16u : 2u * N, S); }
+static inline bool adaptive_stop(uint32_t k, uint32_t N, uint32_t S) {
+    const uint64_t rhs = g_plant == PLANT_ADAPTIVE_STOP ? (uint64_t)N * N : (uint64_t)N * N * N;
+    return N >= S || 4ull * S * k * (N - k) <= rhs;
 }
 static inline bool hard_light(const Frame& f, uint32_t i) {
     const hr_light& lt = f.lights[i];
@@ -1337,6 +1370,11 @@ void or_glass_hit(const float* N, const float* rd, float ior, float* out) {
     v3 d = glass_hit_dir(ld3(N), ld3(rd), ior);
     out[0] = d.x; out[1] = d.y; out[2] = d.z;
 }
+void or_nested_cell(uint32_t s, uint32_t S, uint32_t* cx, uint32_t* nx, uint32_t* cy, uint32_t* ny) {
+    nested_cell(s, S, *cx, *nx, *cy, *ny);
+}
+int or_adaptive_stop(uint32_t k, uint32_t N, uint32_t S) { return adaptive_stop(k, N, S) ? 1 : 0; }
+uint32_t or_adaptive_next(uint32_t N, uint32_t S) { return adaptive_next(N, S); }
 int or_intersect_tri(const float* org, const float* dir, float tmin, float tmax, const float* v0,
                      const float* v1, const float* v2, float* t, float* u, float* v) {
     RayP r = prep_ray(ld3(org), ld3(dir), tmin, tmax);
@@ -1634,27 +1672,23 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
                 float NdotL = clamp01(dot(N, L));
                 float factor = 0.0f;
                 if (NdotL > 0.0f && f.adaptive && (isDir || dist < maxd)) {
-                    /* 1 ray for hard lights, else 4 probes; shadow_samples rays when they disagree */
-                    uint32_t target = hard_light(f, i) ? 1u : 4u;
-                    bool refine = false;
+                    /* 1 ray for a hard light; else the stages of the nested
+                     * sequence, each adding to the samples before it */
+                    const bool hard = hard_light(f, i);
+                    uint32_t target = hard ? 1u : adaptive_first(S), unocc = 0, sI = 0;
                     for (;;) {
-                        uint32_t unocc = 0;
-                        for (uint32_t sI = 0; sI < target; ++sI) {
-                            Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, i * 128u + (refine ? 64u + sI : sI));
-                            v3 dir = light_sample_dir_strat(f, i, P, L, g, sI, target);
+                        for (; sI < target; ++sI) {
+                            Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, i * 128u + sI);
+                            v3 dir = light_sample_dir_nested(f, i, P, L, g, sI, hard ? 1u : S);
                             float tmax = isDir ? 1000.0f : dist + 1.0f;
                             ++c.shadow;
                             bool sh = occluded(*s, add(P, scl(dir, 1e-1f)), dir, 0.001f, tmax, c);
                             if (sh) ++c.occluded; else ++unocc;
                         }
-                        if (!refine && target == 4u && unocc != 0u && unocc != 4u && S > 4u) {
-                            refine = true;
-                            target = S;
-                            continue;
-                        }
-                        factor = (float)unocc / (float)target;
-                        break;
+                        if (hard || adaptive_stop(unocc, target, S)) break;
+                        target = adaptive_next(target, S);
                     }
+                    factor = (float)unocc / (float)target;
                 } else if (NdotL > 0.0f) {
                     for (uint32_t sI = 0; sI < S; ++sI) {
                         bool shadowed;
@@ -1677,6 +1711,39 @@ void or_trace_shadows(const or_scene* s, uint32_t W, uint32_t H, const float* po
         }
     }, total);
     to_counters(total, cnt);
+}
+
+/* TESTS ONLY: the ray that or_trace_shadows traces for sample sI of light li
+ * at pixel (x, y) with adaptive sampling (the same preconditions and set-up
+ * as its adaptive branch). Returns 0 when the factor is set without rays
+ * (background, glass, facing away, out of reach), 1 for a light sampled by
+ * the nested sequence of shadow_samples, 2 for a hard light (one sample). */
+int or_adaptive_shadow_ray(const or_scene* s, uint32_t W, uint32_t H, const float* position, const float* normal,
+                           const hr_light* lights, const hr_light_shape* shapes, uint32_t nl, const hr_camera* cam,
+                           const hr_trace_params* prm, uint32_t x, uint32_t y, uint32_t li, uint32_t sI, hr_ray* ray) {
+    Frame f{lights, shapes, nl, prm->shadow_samples, prm->refl_spp, prm->max_depth, W, H, frame_index(cam), true};
+    size_t p = (size_t)y * W + x;
+    const float* P4 = position + 4 * p;
+    const float* N4 = normal + 4 * p;
+    if (P4[3] == 0.0f && N4[3] == 0.0f) return 0;
+    if ((int)s->mats[mat_index(*s, P4[3])].shading_metal_rough_idx[0] == 4) return 0;
+    v3 P = ld3(P4);
+    v3 N = normalize(ld3(N4));
+    const hr_light& lt = lights[li];
+    bool isDir = lt.pos_maxdist[3] < 0.0f;
+    v3 L = isDir ? ld3(lt.pos_maxdist) : sub(ld3(lt.pos_maxdist), P);
+    float dist = length(L);
+    L = normalize(L);
+    if (!(clamp01(dot(N, L)) > 0.0f) || !(isDir || dist < lt.pos_maxdist[3])) return 0;
+    const bool hard = hard_light(f, li);
+    Pcg g = ray_stream(tea(y * W + x, f.frame), PURPOSE_ADAPTIVE_SHADOW, li * 128u + sI);
+    v3 dir = light_sample_dir_nested(f, li, P, L, g, sI, hard ? 1u : f.samples);
+    v3 o = add(P, scl(dir, 1e-1f));
+    ray->org[0] = o.x; ray->org[1] = o.y; ray->org[2] = o.z;
+    ray->dir[0] = dir.x; ray->dir[1] = dir.y; ray->dir[2] = dir.z;
+    ray->tmin = 0.001f;
+    ray->tmax = isDir ? 1000.0f : dist + 1.0f;
+    return hard ? 2 : 1;
 }
 
 /* hybridRaygen.rgen:27-186 with the bounce chain of :153-170 */

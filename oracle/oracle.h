@@ -56,6 +56,17 @@ void or_sample_rect(const float* centre, const hr_light_shape* shape, const floa
                     float r1, float r2, float* out3);
 void or_glass_raygen(const float* N, const float* V, float ior, float* out3);
 void or_glass_hit(const float* N, const float* rd, float ior, float* out3);
+/* adaptive shadow sampling: the cell (cx of nx columns, cy of ny rows) of
+ * sample s of a nested sequence of S, the stopping rule and the next stage */
+void or_nested_cell(uint32_t s, uint32_t S, uint32_t* cx, uint32_t* nx, uint32_t* cy, uint32_t* ny);
+int or_adaptive_stop(uint32_t k, uint32_t N, uint32_t S);
+/* the ray or_trace_shadows traces for adaptive sample sI of light li at pixel
+ * (x, y): 0 = none (factor set without rays), 1 = sampled light, 2 = hard light */
+int or_adaptive_shadow_ray(const or_scene* s, uint32_t W, uint32_t H, const float* position, const float* normal,
+                           const hr_light* lights, const hr_light_shape* shapes, uint32_t nl, const hr_camera* cam,
+                           const hr_trace_params* params, uint32_t x, uint32_t y, uint32_t li, uint32_t sI,
+                           hr_ray* ray);
+uint32_t or_adaptive_next(uint32_t N, uint32_t S);
 /* TESTS ONLY: plant one deliberate mistake (see PLANT_* in hr_oracle.cpp;
  * 0 restores the faithful oracle) to show that a check catches it. */
 void or_set_plant(int plant);

@@ -92,6 +92,13 @@ def lib():
         L.or_glass_raygen.argtypes = [vp, vp, f32, vp]
         L.or_glass_hit.argtypes = [vp, vp, f32, vp]
         L.or_set_plant.argtypes = [i32]
+        L.or_nested_cell.argtypes = [u32, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+        L.or_adaptive_stop.restype = i32
+        L.or_adaptive_stop.argtypes = [u32, u32, u32]
+        L.or_adaptive_next.restype = u32
+        L.or_adaptive_next.argtypes = [u32, u32]
+        L.or_adaptive_shadow_ray.restype = i32
+        L.or_adaptive_shadow_ray.argtypes = [vp, u32, u32, vp, vp, vp, vp, u32, vp, vp, u32, u32, u32, u32, vp]
         L.or_intersect_tri.restype = i32
         L.or_intersect_tri.argtypes = [vp, vp, f32, f32, vp, vp, vp, vp, vp, vp]
         L.or_scene_build.restype = i32
@@ -250,6 +257,24 @@ class OracleScene:
                               refl.ctypes.data, rows[0], rows[1], C.byref(cnt), self.threads)
         return lit, refl, cnt.as_dict()
 
+    def adaptive_shadow_rays(self, gb, items, params=None):
+        """kinds, rays of adaptive samples items = [(x, y, light, s), ...]:
+        kind 0 = no ray, 1 = sampled light, 2 = hard light"""
+        sc = self.scene
+        la, sa = abi.pack_lights(sc.lights, sc.shapes)
+        cam = abi.pack_camera(sc.camera)
+        prm = self._params(params)
+        rays = np.zeros(len(items), dtype=abi.RAY_DTYPE)
+        kinds = np.zeros(len(items), dtype=np.int32)
+        f = lib().or_adaptive_shadow_ray
+        args = (self.h, sc.width, sc.height, gb["position"].ctypes.data, gb["normal"].ctypes.data,
+                C.cast(la, C.c_void_p), None if sa is None else C.cast(sa, C.c_void_p), len(sc.lights),
+                C.byref(cam), C.byref(prm))
+        base, step = rays.ctypes.data, abi.RAY_DTYPE.itemsize
+        for n, (x, y, li, s) in enumerate(items):
+            kinds[n] = f(*args, int(x), int(y), int(li), int(s), base + n * step)
+        return kinds, rays
+
     def render_rt(self, accum=None, camera=None, params=None, rows=(0, 1), lights=None, shapes="scene"):
         """pure ray tracing mode (raygen.rgen + closesthit.rchit) into accum
         (float32 H x W x 4, updated in place; a new array when None)"""
@@ -384,6 +409,7 @@ class planted:
     """Context manager (tests only): the oracle with one deliberate mistake
     (PLANT_* in hr_oracle.cpp) for the duration of the block."""
     GLASS_ETA, GLASS_NORMAL, FRESNEL_NDOTH, GGX_ALPHA, RECT_STRATA, TIR_REFRACT = 1, 2, 3, 4, 5, 6
+    ADAPTIVE_STOP, NESTED_ORDER = 7, 8
 
     def __init__(self, which):
         self.which = which
@@ -395,6 +421,20 @@ class planted:
     def __exit__(self, *exc):
         lib().or_set_plant(0)
         return False
+
+
+def nested_cell(s, S):
+    v = [C.c_uint32() for _ in range(4)]
+    lib().or_nested_cell(s, S, *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)  # (cx, nx, cy, ny)
+
+
+def adaptive_stop(k, N, S):
+    return bool(lib().or_adaptive_stop(k, N, S))
+
+
+def adaptive_next(N, S):
+    return lib().or_adaptive_next(N, S)
 
 
 def intersect_tri(org, d, tmin, tmax, v0, v1, v2):

@@ -3,7 +3,7 @@
 - HR_FLAG_SHADOW_R8: the shadow pass stores rint(clamp(f, 0, 1) * 255) of the
   factor it computes in FP32 (imageStore to R8_UNORM, shadowRaygen.rgen:93):
   bit-exact against the quantised FP32 factors, which equal the oracle's; on
-  every shadow kernel path (sample-parallel, adaptive probe / refine, group
+  every shadow kernel path (sample-parallel, adaptive stages, group
   per lane, batches);
 - the hybrid pass then reads code / 255 (imageLoad, hybridRaygen.rgen:83): lit
   within 1e-4 of the oracle shading from the same quantised factors;

@@ -152,27 +152,24 @@ def test_frame_index_changes_stream_only_for_sampled_lights(oracle_lib):
 
 
 def test_adaptive_sampling_counts(oracle_lib):
-    """Adaptive sampling: per (pixel, light) 4 probe rays, then 64 more when
-    the probes disagree, so factors are 0, 1 or k/64 and the ray count is
-    4 * groups + 64 * penumbra groups; glossy pixels trace 1 + floor(63 r)
-    reflection paths."""
+    """Adaptive sampling: factors are k / N with N one of the stages 4, 16,
+    32, 64, penumbra groups take more than the 4 probes, and the frame costs
+    fewer rays than 64 fixed samples; glossy pixels trace 1 + floor(63 r)
+    reflection paths at most."""
     sc = scenes.make_terrain_scene(seed=4, width=80, height=45, quads=48, instances=24, soup_tris=128,
                                    samples=64, spp=64, rough_max=1.0, adaptive=True)
     os_ = oracle_lib.OracleScene(sc)
     gb, _ = os_.gbuffer()
     sh, cnt = os_.trace_shadows(gb)
-    vals = np.unique(sh)
-    assert np.all(np.isin(vals, np.arange(65, dtype=np.float32) / np.float32(64)))
+    allowed = np.unique(np.concatenate([np.arange(N + 1, dtype=np.float32) / np.float32(N)
+                                        for N in (4, 16, 32, 64)]))
+    assert np.all(np.isin(np.unique(sh), allowed))
     soft = (sh > 0) & (sh < 1)
     assert soft.sum() > 0
-    # rays: every traced group costs 4; penumbra groups 64 more. Groups whose
-    # light faces away or is the background cost nothing (factor 0).
-    lit_or_shadowed = (sh == 1) | (sh == 0)
-    assert cnt["shadow_rays"] >= 64 * int(soft.sum()) + 4 * int(soft.sum())
-    assert (cnt["shadow_rays"] - 68 * int(soft.sum())) % 4 == 0
-    # non-adaptive at 64 samples traces more rays for the same scene
+    # some penumbra factors need a finer stage than the probes
+    assert not np.all(np.isin(sh[soft], np.arange(5, dtype=np.float32) / np.float32(4)))
     _, c64 = os_.trace_shadows(gb, params=dict(flags=0, shadow_samples=64))
-    assert c64["shadow_rays"] > cnt["shadow_rays"]
+    assert c64["shadow_rays"] > cnt["shadow_rays"] > 0
     lit, refl, c2 = os_.shade_hybrid(gb, sh)
     rough = gb["material"][..., 1]
     glossy = (gb["normal"][..., 3] == 1) & (rough > 0)
@@ -181,7 +178,7 @@ def test_adaptive_sampling_counts(oracle_lib):
     glossy &= modes == 3
     want_max = int(np.sum(1 + np.floor(np.float32(63) * rough[glossy]).astype(np.int64)))
     assert 0 < c2["reflection_rays"] <= want_max
-    assert np.isfinite(lit).all() and lit_or_shadowed.sum() > 0
+    assert np.isfinite(lit).all()
 
 
 @pytest.mark.parametrize("samples", [1, 5, 16])

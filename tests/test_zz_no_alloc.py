@@ -3,7 +3,7 @@ device array) against hr_trace_shadows, through the C ABI.
 
 Bar: bit-exact shadow factors and equal ray counts, on calls in a row with
 shrinking inputs (the array keeps the previous call's contents), on each
-kernel path (sample-parallel, adaptive probe/refine, group per lane); frames
+kernel path (sample-parallel, adaptive stages, group per lane); frames
 whose scratch exceeds HR_NOALLOC_SCRATCH_BYTES are refused on the host.
 """
 import ctypes as C
@@ -30,7 +30,7 @@ def _noalloc(rd, stats):
 
 CASES = [  # (width, height, samples, adaptive): largest first, each call smaller than the one before
     (320, 180, 16, False),   # sample-parallel kernel
-    (200, 120, 32, True),    # adaptive: probes, decide, refine
+    (200, 120, 32, True),    # adaptive: probes, then stages of 16 and 32
     (160, 90, 8, False),
     (96, 54, 5, False),      # a count that is no power of two: group-per-lane kernel
 ]

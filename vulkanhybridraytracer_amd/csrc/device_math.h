@@ -190,7 +190,7 @@ enum : uint32_t {
     PURPOSE_REFLECT = 2,         // index = spp sample
     PURPOSE_HIT_SHADOW = 3,      // index = (sample * max_depth + depth) * nl + light
     PURPOSE_HIT_REFLECT = 4,     // index = sample * max_depth + depth
-    PURPOSE_ADAPTIVE_SHADOW = 5,  // index = light * 128 + k (k < 4 probes, 64 + s refinement)
+    PURPOSE_ADAPTIVE_SHADOW = 5,  // index = light * 128 + s (s: sample of the nested sequence, < 64)
     PURPOSE_RT_PIXEL = 6,         // pure ray tracing: index = pixel sample (sub-pixel offset)
     PURPOSE_RT_SHADOW = 7,        // pure ray tracing: index = ((sample * max_depth + depth) * nl + light)
     PURPOSE_RT_REFLECT = 8        // pure ray tracing: index = sample * max_depth + depth

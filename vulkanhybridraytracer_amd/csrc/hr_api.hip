@@ -171,8 +171,8 @@ hr_status fill_lights(hr_context* ctx, FrameArgs& f, const hr_light* lights, con
     if (!prm) return fail(ctx, HR_E_INVALID, "trace params are NULL");
     if (prm->shadow_samples == 0 || prm->shadow_samples > 4096)
         return fail(ctx, HR_E_INVALID, "shadow_samples must be in 1..4096");
-    if ((prm->flags & HR_FLAG_ADAPTIVE) && prm->shadow_samples > 64)
-        return fail(ctx, HR_E_INVALID, "adaptive sampling takes at most 64 penumbra samples");
+    if ((prm->flags & HR_FLAG_ADAPTIVE) && (prm->shadow_samples > 64 || (prm->shadow_samples & (prm->shadow_samples - 1u))))
+        return fail(ctx, HR_E_INVALID, "adaptive sampling takes a power of two of at most 64 shadow samples");
     if (prm->refl_spp > 4096 || prm->max_depth > 64)
         return fail(ctx, HR_E_INVALID, "refl_spp must be <= 4096 and max_depth <= 64");
     for (uint32_t i = 0; i < nl; ++i) {

@@ -384,8 +384,9 @@ struct RtItems {
     const STri* tris;
     uint32_t node_count;
 
-    __device__ __forceinline__ int setup(uint32_t item, RayP& r, float& tmax, const TNode4*&, const STri*&,
-                                         uint32_t&, Cnt& c) const {
+    __device__ __forceinline__ int setup(uint32_t item, uint32_t& bit, RayP& r, float& tmax, const TNode4*&,
+                                         const STri*&, uint32_t&, Cnt& c) const {
+        bit = item;
         const uint32_t G = item >> lg2s, a = item & ((1u << lg2s) - 1u);
         const uint32_t SS = F.samples;
         if (a >= SS) return ITEM_SKIP;

@@ -203,8 +203,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     size_t out_idx = 0;
     uint32_t out_fmt = 0;
     uint32_t li = 0, seed = 0, s = 0, unocc = 0, SS = 1;
-    uint32_t target = 1;    // rays of the current phase
-    bool refine = false;    // adaptive: the penumbra phase after disagreeing probes
+    uint32_t target = 1;    // rays of the current stage (adaptive) or all rays
+    bool hard = false;      // adaptive: a light of one sample
     v3 P = mk(0.0f, 0.0f, 0.0f), L = P;
     float dist = 0.0f;
     bool isDir = false, in_group = false;
@@ -233,12 +233,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     if (hit) ++c.occluded; else ++unocc;
                 }
                 if (s == target) {
-                    bool adaptive = jobs[job].F.adaptive != 0u;
-                    if (adaptive && !refine && target == 4u && unocc != 0u && unocc != 4u && SS > 4u) {
-                        refine = true;  // the probes disagree: a penumbra
-                        target = SS;
-                        s = 0;
-                        unocc = 0;
+                    // adaptive: the next stage of the nested sequence while the
+                    // estimate is not yet as tight as S samples at one half
+                    if (jobs[job].F.adaptive != 0u && !hard && !adaptive_stop(unocc, target, SS)) {
+                        target = adaptive_next(target, SS);
                     } else {
                         put_factor(out_base, out_idx, (float)unocc / (float)target, out_fmt);
                         in_group = false;
@@ -324,8 +322,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
                     SS = F.samples;
                     s = 0;
                     unocc = 0;
-                    refine = false;
-                    target = F.adaptive ? (hard_light(F, li) ? 1u : 4u) : SS;
+                    hard = F.adaptive && hard_light(F, li);
+                    target = F.adaptive ? (hard ? 1u : adaptive_first(SS)) : SS;
                     if (BATCH) {
                         nodes = J.S.nodes;
                         nodes4 = J.S.nodes4;
@@ -340,8 +338,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
             const FrameArgs& F = jobs[job].F;
             v3 dir;
             if (F.adaptive) {
-                Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (refine ? 64u + s : s));
-                dir = light_sample_dir_strat(F, li, P, L, g, s, target);
+                Pcg g = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + s);
+                dir = light_sample_dir_nested(F, li, P, L, g, s, hard ? 1u : SS);
             } else {
                 Pcg g = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
                 dir = light_sample_dir(F, li, P, L, g, s, SS);
@@ -520,11 +518,14 @@ inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u
 // each group's bits into the factor count / S. Items are fetched from the
 // same per-XCD bands as in k_shadows.
 // MODE 0: S samples per group (no adaptive sampling). With adaptive sampling
-// the same kernel runs twice: MODE 1 traces the probes (4 per group, 1 for a
-// hard light; items numbered with 4 per group), k_shadow_decide settles the
-// groups whose probes agree and lists the others, and MODE 2 traces S refine
-// samples for each listed group (item = list slot * S + sample; the item count
-// is read from *list_count on the device).
+// (the nested sequence of adaptive_stop / adaptive_next) each group owns a
+// window of 2^lg2s bits, bit s for sample s, that every stage adds to: MODE 1
+// traces the probes (items numbered 4 per group; 1 probe for a hard light),
+// k_adaptive_decide<true> settles the groups whose estimate is tight enough and
+// lists the others, and MODE 2 traces a stage's samples [n0, n0 + size) for
+// each listed group (item = list slot * size + sample - n0; the item count is
+// read from *list_count on the device), after which k_adaptive_decide<false>
+// settles or lists the groups again, up to 2^lg2s samples.
 // Several frames (njobs > 1): groups are numbered frame after frame (the
 // frames' group_base), every frame uses the stride 2^lg2s (the largest sample
 // count) and skips samples at or beyond its own count.
@@ -541,20 +542,32 @@ __device__ __forceinline__ uint32_t job_of(const ShadowJob* __restrict__ jobs, u
 template <int MODE, bool MULTI>
 struct HybridItems {
     const ShadowJob* __restrict__ jobs;
-    uint32_t njobs, ilg;
+    uint32_t njobs, lg2s;
     const uint32_t* __restrict__ list;
+    uint32_t n0, size;  // MODE 2: the stage's first sample and sample count
     const TNode4* nodes4;
     const STri* tris;
     uint32_t node_count;
 
-    __device__ __forceinline__ int setup(uint32_t item, RayP& r, float& tmax, const TNode4*& n4, const STri*& tr,
-                                         uint32_t& nc, Cnt& c) const {
-        const uint32_t G = MODE == 2 ? list[item >> ilg] : item >> ilg;  // global group
-        const uint32_t s = item & ((1u << ilg) - 1u);
+    __device__ __forceinline__ int setup(uint32_t item, uint32_t& bit, RayP& r, float& tmax, const TNode4*& n4,
+                                         const STri*& tr, uint32_t& nc, Cnt& c) const {
+        uint32_t G, s;  // global group, sample
+        if (MODE == 0) {
+            G = item >> lg2s;
+            s = item & ((1u << lg2s) - 1u);
+        } else if (MODE == 1) {
+            G = item >> 2;
+            s = item & 3u;
+        } else {
+            const uint32_t slot = item / size;
+            G = list[slot];
+            s = n0 + (item - slot * size);
+        }
+        bit = MODE == 0 ? item : (G << lg2s) | s;
         const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
         const FrameArgs& F = J.F;
         const uint32_t SS = F.samples;
-        if (MULTI && MODE != 1 && s >= SS) return ITEM_SKIP;  // beyond this frame's sample count
+        if (s >= SS) return ITEM_SKIP;  // beyond this frame's sample count
         const uint32_t g = G - J.group_base;
         const uint32_t per_tile = F.nl * 64u;
         const uint32_t t = g / per_tile, rem = g - t * per_tile;
@@ -562,7 +575,7 @@ struct HybridItems {
         uint32_t x, y;
         tile_pixel(J, t, q, x, y);
         if (x >= F.width || y >= F.height) return ITEM_SKIP;
-        const bool hard = MODE == 1 && hard_light(F, li);
+        const bool hard = MODE != 0 && hard_light(F, li);
         if (hard && s > 0u) return ITEM_SKIP;  // a hard light takes one probe
         const size_t p = (size_t)y * F.width + x;
         const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
@@ -585,9 +598,9 @@ struct HybridItems {
         if (MODE == 0) {
             Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
             dir = light_sample_dir(F, li, P, L, gs, s, SS);
-        } else {  // adaptive: probes at index li * 128 + s, refine samples at li * 128 + 64 + s
-            Pcg gs = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + (MODE == 2 ? 64u + s : s));
-            dir = light_sample_dir_strat(F, li, P, L, gs, s, MODE == 2 ? SS : (hard ? 1u : 4u));
+        } else {  // adaptive: sample s of the light's nested sequence
+            Pcg gs = ray_stream(seed, PURPOSE_ADAPTIVE_SHADOW, li * 128u + s);
+            dir = light_sample_dir_nested(F, li, P, L, gs, s, hard ? 1u : SS);
         }
         tmax = isDir ? 1000.0f : dist + 1.0f;
         r = prep_ray(add(P, scl(dir, 1e-1f)), dir);
@@ -606,13 +619,13 @@ struct HybridItems {
 template <int MODE, bool MULTI, bool OFF32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
-    uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
-    unsigned long long* counters) {
+    uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, uint32_t n0,
+    uint32_t size, unsigned long long* counters) {
     static_assert(!(MULTI && OFF32), "frames of a batch have their own node arrays");
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
-    if (MODE == 2) nitems = *list_count << lg2s;
-    HybridItems<MODE, MULTI> src{jobs, njobs, MODE == 1 ? 2u : lg2s, list, jobs[0].S.nodes4, jobs[0].S.tris,
+    if (MODE == 2) nitems = *list_count * size;
+    HybridItems<MODE, MULTI> src{jobs, njobs, lg2s, list, n0, size, jobs[0].S.nodes4, jobs[0].S.tris,
                                  jobs[0].S.node_count};
     item_loop<OFF32>(src, nitems, next_item, bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
@@ -622,16 +635,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
 template <int MODE>
 void launch_spl_mode(bool multi, bool off32, uint32_t nb, hipStream_t st, const ShadowJob* jobs, uint32_t njobs,
                      uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits, const uint32_t* list,
-                     const uint32_t* list_count, unsigned long long* counters) {
+                     const uint32_t* list_count, uint32_t n0, uint32_t size, unsigned long long* counters) {
     if (multi)
         k_shadows_spl<MODE, true, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                               list_count, counters);
+                                                               list_count, n0, size, counters);
     else if (off32)
         k_shadows_spl<MODE, false, true><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                               list_count, counters);
+                                                               list_count, n0, size, counters);
     else
         k_shadows_spl<MODE, false, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                                list_count, counters);
+                                                                list_count, n0, size, counters);
 }
 
 // shadow factor of every group from the bits of its samples
@@ -654,14 +667,33 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
                F.unorm8);
 }
 
-// Adaptive sampling, after the probes: a hard light's factor is its one
-// probe; 4 agreeing probes (or S <= 4) give count / 4; the other groups are
-// appended to the refine list.
-__global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restrict__ jobs, uint32_t njobs,
-                                                       uint32_t ngroups, const uint32_t* __restrict__ bits,
-                                                       uint32_t* list, uint32_t* list_count) {
-    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
-    if (G >= ngroups) return;
+// unoccluded samples among the first N (<= 2^lg2s <= 64) of the window at bit b0
+__device__ __forceinline__ uint32_t prefix_bits(const uint32_t* __restrict__ bits, uint64_t b0, uint32_t lg2s,
+                                                uint32_t N) {
+    auto mask = [](uint32_t n) { return n >= 32u ? 0xFFFFFFFFu : (1u << n) - 1u; };
+    if (lg2s >= 5u) {
+        uint32_t k = (uint32_t)__popc(bits[b0 >> 5] & mask(N));
+        if (N > 32u) k += (uint32_t)__popc(bits[(b0 >> 5) + 1u] & mask(N - 32u));
+        return k;
+    }
+    return (uint32_t)__popc((bits[b0 >> 5] >> (uint32_t)(b0 & 31u)) & mask(N));
+}
+
+// Adaptive sampling between stages. FIRST: every group after its probes (a
+// hard light's factor is its one probe); else the groups of list_in after the
+// stage that ended at sample n_end. A group whose estimate k / N is tight
+// enough (adaptive_stop, N = min(n_end, its frame's S)) gets its factor, the
+// others go to list_out for the next stage.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_adaptive_decide(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                         uint32_t ngroups, uint32_t lg2s,
+                                                         const uint32_t* __restrict__ bits,
+                                                         const uint32_t* __restrict__ list_in,
+                                                         const uint32_t* __restrict__ count_in, uint32_t n_end,
+                                                         uint32_t* list_out, uint32_t* count_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (FIRST ? ngroups : *count_in)) return;
+    const uint32_t G = FIRST ? i : list_in[i];
     const ShadowJob& J = jobs[njobs > 1u ? job_of(jobs, njobs, G) : 0u];
     const FrameArgs& F = J.F;
     const uint32_t SS = F.samples;
@@ -672,41 +704,19 @@ __global__ __launch_bounds__(256) void k_shadow_decide(const ShadowJob* __restri
     uint32_t x, y;
     tile_pixel(J, t, q, x, y);
     if (x >= F.width || y >= F.height) return;
-    const uint32_t w = (bits[G >> 3] >> ((G & 7u) * 4u)) & 15u;  // the 4 probe bits of the group
-    float* out_base = J.shadow;
-    const size_t out_idx = li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
-    const uint32_t out_fmt = F.unorm8;
-    if (hard_light(F, li)) {
-        put_factor(out_base, out_idx, (float)(w & 1u) / 1.0f, out_fmt);
+    const size_t idx = li * ((size_t)F.width * F.height) + (size_t)y * F.width + x;
+    const uint64_t b0 = (uint64_t)G << lg2s;
+    if (FIRST && hard_light(F, li)) {
+        put_factor(J.shadow, idx, (float)prefix_bits(bits, b0, lg2s, 1u) / 1.0f, F.unorm8);
         return;
     }
-    const uint32_t n = (uint32_t)__popc(w);
-    if (n != 0u && n != 4u && SS > 4u) {
-        list[atomicAdd(list_count, 1u)] = G;
+    const uint32_t N = FIRST ? adaptive_first(SS) : min(n_end, SS);
+    const uint32_t k = prefix_bits(bits, b0, lg2s, N);
+    if (adaptive_stop(k, N, SS)) {
+        put_factor(J.shadow, idx, (float)k / (float)N, F.unorm8);
         return;
     }
-    put_factor(out_base, out_idx, (float)n / 4.0f, out_fmt);
-}
-
-// factors of the refined groups
-__global__ __launch_bounds__(256) void k_shadow_refine_bits(const ShadowJob* __restrict__ jobs, uint32_t njobs,
-                                                            uint32_t lg2s, const uint32_t* __restrict__ bits,
-                                                            const uint32_t* __restrict__ list,
-                                                            const uint32_t* __restrict__ list_count) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= *list_count) return;
-    const uint32_t G = list[slot];
-    const ShadowJob& J = jobs[njobs > 1u ? job_of(jobs, njobs, G) : 0u];
-    const FrameArgs& F = J.F;
-    const uint32_t g = G - J.group_base;
-    const uint32_t per_tile = F.nl * 64u;
-    const uint32_t t = g / per_tile, rem = g - t * per_tile;
-    const uint32_t li = rem >> 6, q = rem & 63u;
-    uint32_t x, y;
-    tile_pixel(J, t, q, x, y);
-    const uint32_t n = window_bits(bits, (uint64_t)slot << lg2s, lg2s);
-    put_factor(J.shadow, li * ((size_t)F.width * F.height) + (size_t)y * F.width + x, (float)n / (float)F.samples,
-               F.unorm8);
+    list_out[atomicAdd(count_out, 1u)] = G;
 }
 
 }  // namespace
@@ -731,10 +741,17 @@ uint32_t shadow_groups(const FrameArgs& f, uint32_t tile_w) {
     return tiles * f.nl * 64u;
 }
 
+// the stage after one that ended at sample N of at most S (adaptive_next)
+static uint32_t host_adaptive_next(uint32_t N, uint32_t S) {
+    const uint32_t n = N == 4u ? 16u : 2u * N;
+    return n < S ? n : S;
+}
+
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
     if (!adaptive) return w;
-    return w + ((((size_t)ngroups << 2) + 31u) >> 5) * 4u + (size_t)ngroups * 4u + 64u;
+    // bits, two lists of groups, their two counts on lines of their own
+    return w + 2u * (size_t)ngroups * 4u + 2u * 256u;
 }
 
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
@@ -748,31 +765,41 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     uint32_t nb = min(resident_blocks, (uint32_t)((items + BLOCK - 1) / BLOCK));
     if (nb == 0) return hipSuccess;
     uint32_t* bits = scratch;
+    k_reset_u32<<<1, 64, 0, st>>>(next_item);
+    k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
     if (!adaptive) {
-        k_reset_u32<<<1, 64, 0, st>>>(next_item);
-        k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
         launch_spl_mode<0>(multi, off32, nb, st, jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits, nullptr,
-                           nullptr, counters);
+                           nullptr, 0u, 0u, counters);
         k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
-    // scratch: refine bits (S per group at most), probe bits (4 per group), list, count
-    uint32_t* pbits = bits + words;
-    const uint32_t pwords = (uint32_t)((((uint64_t)ngroups << 2) + 31u) >> 5);
-    uint32_t* list = pbits + pwords;
-    uint32_t* count = list + ngroups;
+    // scratch: bits (2^lg2s per group, every stage adds its own), lists A and
+    // B, their counts 64 words apart
+    uint32_t* lists[2] = {bits + words, bits + words + ngroups};
+    uint32_t* counts[2] = {bits + words + 2u * (size_t)ngroups, bits + words + 2u * (size_t)ngroups + 64u};
+    const uint32_t S = 1u << lg2s;
+    const uint32_t gb = (ngroups + 255u) / 256u;
+    k_fill_bits<<<1, 64, 0, st>>>(counts[0], 1u);
     const uint32_t pitems = ngroups << 2;
-    const uint32_t nbp = min(resident_blocks, (pitems + BLOCK - 1) / BLOCK);
-    k_reset_u32<<<1, 64, 0, st>>>(next_item);
-    k_fill_bits<<<(pwords + 255u) / 256u, 256, 0, st>>>(pbits, pwords);
-    k_fill_bits<<<1, 64, 0, st>>>(count, 1u);
-    launch_spl_mode<1>(multi, off32, nbp, st, jobs_dev, njobs, pitems, lg2s, next_item, pbits, nullptr, nullptr,
-                       counters);
-    k_shadow_decide<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, pbits, list, count);
-    k_reset_u32<<<1, 64, 0, st>>>(next_item);
-    k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
-    launch_spl_mode<2>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, counters);
-    k_shadow_refine_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, lg2s, bits, list, count);
+    launch_spl_mode<1>(multi, off32, min(resident_blocks, (pitems + BLOCK - 1) / BLOCK), st, jobs_dev, njobs, pitems,
+                       lg2s, next_item, bits, nullptr, nullptr, 0u, 4u, counters);
+    k_adaptive_decide<true><<<gb, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, nullptr, nullptr, 4u,
+                                                lists[0], counts[0]);
+    int cur = 0;
+    for (uint32_t N = 4u; N < S;) {
+        const uint32_t Nn = host_adaptive_next(N, S);
+        const uint32_t size = Nn - N;
+        const uint64_t most = (uint64_t)ngroups * size;
+        k_reset_u32<<<1, 64, 0, st>>>(next_item);
+        k_fill_bits<<<1, 64, 0, st>>>(counts[cur ^ 1], 1u);
+        const uint64_t want = (most + BLOCK - 1) / BLOCK;
+        launch_spl_mode<2>(multi, off32, want < resident_blocks ? (uint32_t)want : resident_blocks, st,
+                           jobs_dev, njobs, 0u, lg2s, next_item, bits, lists[cur], counts[cur], N, size, counters);
+        k_adaptive_decide<false><<<gb, 256, 0, st>>>(jobs_dev, njobs, 0u, lg2s, bits, lists[cur], counts[cur], Nn,
+                                                     lists[cur ^ 1], counts[cur ^ 1]);
+        cur ^= 1;
+        N = Nn;
+    }
     return hipGetLastError();
 }
 

@@ -446,23 +446,54 @@ __device__ __forceinline__ v3 light_sample_dir(const FrameArgs& F, uint32_t i, v
         return sampleRect(ld3p(lt.pos_maxdist), F.shapes[i], P, s, S, r1, r2);
     return sampleDisk(lt, P, L, r1, r2);
 }
-// Adaptive-sampling light sample: cell s of the n x ceil(S/n) grid over the
-// light (n = floor(sqrt(S))), for rectangles as sampleRect and for disks on
-// the (radius, angle) square of sampleDisk.
-__device__ __forceinline__ v3 light_sample_dir_strat(const FrameArgs& F, uint32_t i, v3 P, v3 L, Pcg& g,
-                                                     uint32_t s, uint32_t S) {
+// ---- adaptive shadow sampling --------------------------------------------------
+// Samples of a (pixel, light) come from one nested stratified sequence of S (a
+// power of two <= 64) cells: sample s lies in column cx of nx = 2^ceil(lg S / 2)
+// and row cy of ny = S / nx, where the bits of s at even positions, reversed,
+// give cx and those at odd positions, reversed, give cy. Every prefix of 4^j
+// samples then holds one sample per cell of the 2^j x 2^j coarsening (of the
+// 2^(j+1) x 2^j one for 2 * 4^j), so each stage extends a stratified set.
+__device__ __forceinline__ void nested_cell(uint32_t s, uint32_t S, uint32_t& cx, uint32_t& nx, uint32_t& cy,
+                                            uint32_t& ny) {
+    const uint32_t lg = 31u - (uint32_t)__clz(S);
+    const uint32_t a = (lg + 1u) >> 1, b = lg >> 1;
+    cx = 0u;
+    cy = 0u;
+    for (uint32_t k = 0; k < a; ++k) cx |= ((s >> (2u * k)) & 1u) << (a - 1u - k);
+    for (uint32_t k = 0; k < b; ++k) cy |= ((s >> (2u * k + 1u)) & 1u) << (b - 1u - k);
+    nx = 1u << a;
+    ny = 1u << b;
+}
+// Sample s of the sequence: the cell's point jittered by (r1, r2), on the
+// rectangle (as sampleRect) or on the (radius^2, angle) square of sampleDisk.
+__device__ __forceinline__ v3 light_sample_dir_nested(const FrameArgs& F, uint32_t i, v3 P, v3 L, Pcg& g,
+                                                      uint32_t s, uint32_t S) {
     const hr_light& lt = F.lights[i];
-    float r1 = u01(g);
-    float r2 = u01(g);
-    bool isDir = lt.pos_maxdist[3] < 0.0f;
-    if (!isDir && F.has_shapes && F.shapes[i].type == HR_LIGHT_RECT)
-        return sampleRect(ld3p(lt.pos_maxdist), F.shapes[i], P, s, S, r1, r2);
-    uint32_t n = 1;
-    while ((n + 1) * (n + 1) <= S) ++n;
-    uint32_t m = (S + n - 1) / n;
-    float a = ((float)(s % n) + r1) / (float)n;
-    float b = ((float)(s / n) + r2) / (float)m;
+    const float r1 = u01(g);
+    const float r2 = u01(g);
+    uint32_t cx, nx, cy, ny;
+    nested_cell(s, S, cx, nx, cy, ny);
+    const float a = ((float)cx + r1) / (float)nx;
+    const float b = ((float)cy + r2) / (float)ny;
+    const bool isDir = lt.pos_maxdist[3] < 0.0f;
+    if (!isDir && F.has_shapes && F.shapes[i].type == HR_LIGHT_RECT) {
+        const v3 eu = ld3p(F.shapes[i].edge_u), ev = ld3p(F.shapes[i].edge_v);
+        const v3 target = add(add(ld3p(lt.pos_maxdist), scl(eu, a - 0.5f)), scl(ev, b - 0.5f));
+        return normalize(sub(target, P));
+    }
     return sampleDisk(lt, P, L, a, b);
+}
+// Stages: 4 probes (all S when S < 4), then 16, then doubling up to S.
+__device__ __forceinline__ uint32_t adaptive_first(uint32_t S) { return S < 4u ? S : 4u; }
+__device__ __forceinline__ uint32_t adaptive_next(uint32_t N, uint32_t S) {
+    const uint32_t n = N == 4u ? 16u : 2u * N;
+    return n < S ? n : S;
+}
+// After N samples with k unoccluded: stop when N reaches S or when the
+// binomial variance estimate of the visible fraction, k (N - k) / N^3, is at
+// most that of S samples at one half, 1 / (4 S): exact in integers.
+__device__ __forceinline__ bool adaptive_stop(uint32_t k, uint32_t N, uint32_t S) {
+    return N >= S || 4ull * S * k * (N - k) <= (uint64_t)N * N * N;
 }
 // A light whose adaptive sample count is 1: a disk (or direction) of radius 0.
 __device__ __forceinline__ bool hard_light(const FrameArgs& F, uint32_t i) {
@@ -790,10 +821,10 @@ __device__ __forceinline__ HitGeo hit_geometry(const DevScene& S, v3 ro, v3 rd, 
 // each in next_item; workgroup b starts at band b mod SHADOW_QUEUES and moves
 // to the next band when its own runs dry). Lanes that take work together take
 // consecutive items. A lane whose ray ended takes its next item at once, while
-// the rest of the wave keeps traversing. src.setup(item, ...) decides an item:
-// ITEM_SKIP (no bit), ITEM_LIT (bit set without a ray) or ITEM_TRACE (ray,
-// tmax and scene set); an unoccluded traced item sets bit `item` of `bits`
-// (atomicOr, so the result does not depend on the order).
+// the rest of the wave keeps traversing. src.setup(item, bit, ...) decides an
+// item and the bit of `bits` it owns: ITEM_SKIP (no bit), ITEM_LIT (bit set
+// without a ray) or ITEM_TRACE (ray, tmax and scene set); an unoccluded traced
+// item sets its bit (atomicOr, so the result does not depend on the order).
 enum : int { ITEM_SKIP = 0, ITEM_LIT = 1, ITEM_TRACE = 2 };
 
 // Bit `item` of `bits` for every calling lane with `set`, one global atomicOr
@@ -825,7 +856,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
     const TNode4* nodes4 = src.nodes4;
     const STri* tris = src.tris;
     uint32_t node_count = src.node_count;
-    uint32_t item = 0;
+    uint32_t item = 0, bit = 0;
     bool tracing = false, hit = false;
     uint32_t cur = DONE, leaf = 0u;
     RayP r;
@@ -840,7 +871,7 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
         if (cur == DONE && leaf == 0u) {
             // the ray of `item` ended
             if (tracing && hit) ++c.occluded;
-            or_item_bits(bits, item, tracing && !hit, slot);
+            or_item_bits(bits, bit, tracing && !hit, slot);
             tracing = false;
             bool exhausted = false;
             for (;;) {
@@ -865,8 +896,8 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
                     emptied = k + 1u;
                     continue;
                 }
-                const int kind = src.setup(item, r, tmax, nodes4, tris, node_count, c);
-                or_item_bits(bits, item, kind == ITEM_LIT, slot);
+                const int kind = src.setup(item, bit, r, tmax, nodes4, tris, node_count, c);
+                or_item_bits(bits, bit, kind == ITEM_LIT, slot);
                 if (kind != ITEM_TRACE) continue;
                 if (OFF32) pl = near_planes(r);
                 tracing = true;
