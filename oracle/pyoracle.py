@@ -12,7 +12,8 @@ import numpy as np
 from vulkanhybridraytracer_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle_hr.so")
+# HR_ORACLE_LIB selects another build of the same source (the sanitizer run's)
+LIB_PATH = os.environ.get("HR_ORACLE_LIB") or os.path.join(HERE, "liboracle_hr.so")
 
 
 # A one-GPU box gives a process a share of 16 host threads, whatever the

@@ -269,6 +269,129 @@ def test_gltf_errors(tmp_path):
         assets.decode_image(b"GIF89a....")
 
 
+# ---------------------------------------------------------------- corrupt files
+def _mutants(data, rng, n):
+    """n seeded corruptions of `data`: random byte flips, flips in the header
+    (the first 40 bytes, where sizes live), truncations, and splices of a
+    piece of the file into itself"""
+    out = []
+    for _ in range(n):
+        b = bytearray(data)
+        kind = int(rng.integers(4))
+        if kind in (0, 1):
+            span = len(b) if kind == 0 else min(len(b), 40)
+            for _ in range(int(rng.integers(1, 9))):
+                b[int(rng.integers(span))] = int(rng.integers(256))
+        elif kind == 2:
+            b = b[:int(rng.integers(len(b)))]
+        else:
+            i, j = sorted(int(v) for v in rng.integers(len(b), size=2))
+            k = int(rng.integers(len(b)))
+            b = b[:k] + b[i:j] + b[k:]
+        out.append(bytes(b))
+    return out
+
+
+def _jpeg(w, h, **opts):
+    buf = io.BytesIO()
+    PIL.fromarray(_photo(w, h, 2)).save(buf, "JPEG", quality=85, **opts)
+    return buf.getvalue()
+
+
+CORRUPT_CASES = {
+    "png": lambda: fx.make_png(fx.PNG_CASES[0], 23, 19, 1),
+    "png_interlaced": lambda: fx.make_png(next(c for c in fx.PNG_CASES if c[6]), 23, 19, 1),
+    "jpeg": lambda: _jpeg(41, 29, subsampling=2),
+    "jpeg_progressive": lambda: _jpeg(41, 29, subsampling=2, progressive=True),
+    "hdr_rle": lambda: fx.hdr_bytes(fx.make_rgbe(37, 6, seed=3), rle=True),
+    "hdr_flat": lambda: fx.hdr_bytes(fx.make_rgbe(37, 6, seed=3), rle=False),
+}
+
+
+@pytest.mark.parametrize("fmt", sorted(CORRUPT_CASES))
+def test_decoders_survive_corrupt_files(fmt):
+    """Seeded corruptions of valid files each decode to an RGBA8 image within
+    the size limit or raise AssetError: no crash, no hang, no oversized
+    allocation (the sanitizer run of this suite also checks every access)."""
+    data = CORRUPT_CASES[fmt]()
+    assets.decode_image(data)  # the original decodes
+    rng = np.random.default_rng(sum(map(ord, fmt)))
+    ok = bad = 0
+    for m in _mutants(data, rng, 300):
+        try:
+            img = assets.decode_image(m)
+        except assets.AssetError:
+            bad += 1
+            continue
+        assert img.ndim == 3 and img.shape[2] == 4 and img.shape[0] * img.shape[1] <= 1 << 28
+        ok += 1
+    assert bad > 0
+
+
+def test_decoders_refuse_oversized_headers():
+    """A header claiming more than 2^28 pixels is refused before any buffer
+    of that size exists; so is a claim the remaining bytes cannot back."""
+    with pytest.raises(assets.AssetError, match="limit"):
+        assets.decode_image(fx.png_bytes(np.zeros((1, 1, 3), np.uint8), 2, 8)[:16]
+                            + (40000).to_bytes(4, "big") + (40000).to_bytes(4, "big")
+                            + fx.png_bytes(np.zeros((1, 1, 3), np.uint8), 2, 8)[24:])
+    hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 30000 +X 30000\n" + bytes(64)
+    with pytest.raises(assets.AssetError, match="limit"):
+        assets.decode_image(hdr)
+    hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 9000 +X 9000\n" + bytes(64)
+    with pytest.raises(assets.AssetError, match="truncated"):
+        assets.decode_image(hdr)
+    jpg = bytearray(_jpeg(16, 16))
+    k = jpg.index(b"\xff\xc0")
+    jpg[k + 5:k + 9] = (12000).to_bytes(2, "big") + (12000).to_bytes(2, "big")
+    with pytest.raises(assets.AssetError, match="truncated"):
+        assets.decode_image(bytes(jpg))
+
+
+def test_gltf_loader_survives_corrupt_files(tmp_path):
+    """Seeded corruptions of a .glb and of a .gltf's JSON numbers load or raise
+    AssetError; offsets, counts and strides that would reach outside their
+    buffer (negative, huge, or wrapping around when multiplied) are refused."""
+    (tmp_path / "g").mkdir()
+    (tmp_path / "t").mkdir()
+    glb = fx.make_gltf(str(tmp_path / "g"), glb=True, images="embedded")
+    data = open(glb, "rb").read()
+    rng = np.random.default_rng(7)
+    p = tmp_path / "m.glb"
+    for m in _mutants(data, rng, 150):
+        p.write_bytes(m)
+        try:
+            assets.load_asset(str(p))
+        except assets.AssetError:
+            pass
+    gltf = fx.make_gltf(str(tmp_path / "t"), glb=False, images="data")
+    doc = json.load(open(gltf))
+    acc = doc["accessors"][0]
+    for field, value in (("count", 10 ** 15), ("count", -3), ("byteOffset", -16), ("byteOffset", 10 ** 15)):
+        bad = json.loads(json.dumps(doc))
+        bad["accessors"][0] = dict(acc, **{field: value})
+        q = tmp_path / "t" / ("bad_%s_%d.gltf" % (field, value))
+        q.write_text(json.dumps(bad))
+        with pytest.raises(assets.AssetError):
+            assets.load_asset(str(q))
+    view = dict(doc["bufferViews"][acc["bufferView"]], byteStride=10 ** 15)
+    bad = json.loads(json.dumps(doc))
+    bad["bufferViews"][acc["bufferView"]] = view
+    bad["accessors"][0] = dict(acc, count=2 ** 49 + 1)
+    q = tmp_path / "t" / "bad_stride.gltf"
+    q.write_text(json.dumps(bad))
+    with pytest.raises(assets.AssetError, match="exceeds"):
+        assets.load_asset(str(q))
+    # a subtree shared at every level would multiply without bound
+    bad = json.loads(json.dumps(doc))
+    bad["nodes"] = [{"children": [k + 1, k + 1]} for k in range(40)] + [{"mesh": 0}]
+    bad["scenes"] = [{"nodes": [0]}]
+    q = tmp_path / "t" / "bad_nodes.gltf"
+    q.write_text(json.dumps(bad))
+    with pytest.raises(assets.AssetError, match="node visits"):
+        assets.load_asset(str(q))
+
+
 # ---------------------------------------------------------------- composition
 def test_compose_scene_and_render(tmp_path, oracle_lib):
     path = fx.make_gltf(str(tmp_path))

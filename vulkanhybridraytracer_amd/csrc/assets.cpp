@@ -272,6 +272,25 @@ long req_inum(const JVal& o, const char* k) {
     if (!v) fail("gltf: missing '%s'", k);
     return inum(v, 0);
 }
+// byte offsets, lengths, strides and counts: whole numbers >= 0 (below 1e15)
+size_t unum(const JVal* v, long dflt, const char* what) {
+    const long x = inum(v, dflt);
+    if (x < 0) fail("gltf: negative %s", what);
+    return (size_t)x;
+}
+size_t req_unum(const JVal& o, const char* k) {
+    const long x = req_inum(o, k);
+    if (x < 0) fail("gltf: negative %s", k);
+    return (size_t)x;
+}
+// count elements of elem bytes, stride bytes apart from byte off, all inside
+// a buffer of size bytes (no sum or product below can wrap: each operand is
+// at most 1e15 and the division keeps the last one in range)
+bool span_ok(size_t size, size_t off, size_t count, size_t stride, size_t elem) {
+    if (off > size || elem > size - off) return count == 0 && off <= size;
+    if (count == 0) return true;
+    return stride != 0 && count - 1 <= (size - off - elem) / stride;
+}
 
 // ---- paths, URIs, base64 ------------------------------------------------------------
 std::string dirname_of(const std::string& p) {
@@ -574,12 +593,12 @@ struct Gltf {
         const JVal& view = arr("bufferViews").at((size_t)bv);
         long bi = req_inum(view, "buffer");
         if (bi < 0 || (size_t)bi >= buffers.size()) fail("gltf: buffer %ld out of range", bi);
-        size_t off = (size_t)inum(view.get("byteOffset"), 0) + (size_t)inum(acc.get("byteOffset"), 0);
-        size_t count = (size_t)req_inum(acc, "count");
-        size_t stride = (size_t)inum(view.get("byteStride"), 0);
+        size_t off = unum(view.get("byteOffset"), 0, "byteOffset") + unum(acc.get("byteOffset"), 0, "byteOffset");
+        size_t count = req_unum(acc, "count");
+        size_t stride = unum(view.get("byteStride"), 0, "byteStride");
         if (stride == 0) stride = 4 * (size_t)comps;
         const auto& buf = buffers[(size_t)bi];
-        if (count && off + (count - 1) * stride + 4 * (size_t)comps > buf.size())
+        if (!span_ok(buf.size(), off, count, stride, 4 * (size_t)comps))
             fail("gltf: %s accessor exceeds its buffer", what);
         return View{buf.data() + off, stride, count};
     }
@@ -675,8 +694,9 @@ struct Gltf {
             const JVal& view = arr("bufferViews").at((size_t)bv);
             long bi = req_inum(view, "buffer");
             if (bi < 0 || (size_t)bi >= buffers.size()) fail("gltf: buffer %ld out of range", bi);
-            size_t off = (size_t)inum(view.get("byteOffset"), 0), len = (size_t)req_inum(view, "byteLength");
-            if (off + len > buffers[(size_t)bi].size()) fail("gltf: image buffer view exceeds its buffer");
+            size_t off = unum(view.get("byteOffset"), 0, "byteOffset"), len = req_unum(view, "byteLength");
+            if (!span_ok(buffers[(size_t)bi].size(), off, 1, 1, len))
+                fail("gltf: image buffer view exceeds its buffer");
             bytes.assign(buffers[(size_t)bi].begin() + (long)off, buffers[(size_t)bi].begin() + (long)(off + len));
         }
         hr::img::Image img;
@@ -697,6 +717,10 @@ struct Gltf {
 
     int load_node(long ni, int parent, bool invert, int depth) {
         if (depth > 128) fail("gltf: node hierarchy deeper than 128 (cycle?)");
+        // a node listed as a child more than once is visited once per listing
+        // (the reference instances it each time); a shared subtree repeated
+        // at every level would multiply without bound
+        if (nodes.size() >= (1u << 20)) fail("gltf: more than %u node visits", 1u << 20);
         const JVal& tn = arr("nodes").at((size_t)ni);
         int id = (int)nodes.size();
         nodes.push_back(GNode());
@@ -761,13 +785,13 @@ struct Gltf {
             const JVal& view = arr("bufferViews").at((size_t)bv);
             long bi = req_inum(view, "buffer");
             if (bi < 0 || (size_t)bi >= buffers.size()) fail("gltf: buffer %ld out of range", bi);
-            size_t off = (size_t)inum(view.get("byteOffset"), 0) + (size_t)inum(acc.get("byteOffset"), 0);
-            size_t n = (size_t)req_inum(acc, "count");
+            size_t off = unum(view.get("byteOffset"), 0, "byteOffset") + unum(acc.get("byteOffset"), 0, "byteOffset");
+            size_t n = req_unum(acc, "count");
             long ct = inum(acc.get("componentType"), 0);
             size_t es = ct == 5125 ? 4 : ct == 5123 ? 2 : ct == 5121 ? 1 : 0;
             if (!es) fail("gltf: index component type %ld not supported", ct);
             const auto& buf = buffers[(size_t)bi];
-            if (off + n * es > buf.size()) fail("gltf: index accessor exceeds its buffer");
+            if (!span_ok(buf.size(), off, n, es, es)) fail("gltf: index accessor exceeds its buffer");
             for (size_t i = 0; i < n; ++i) {
                 const uint8_t* q = buf.data() + off + i * es;
                 uint32_t v = es == 4 ? (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24)
@@ -900,6 +924,9 @@ hr_status hr_asset_load(const char* path, uint32_t flags, hr_asset** out, char* 
     } catch (const std::bad_alloc&) {
         set_err(err, err_len, "out of memory");
         return HR_E_OOM;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, std::string("asset load failed: ") + e.what());
+        return HR_E_INVALID;
     }
     for (const auto& im : A->images) A->tex.push_back(hr_texture{im.px.data(), im.w, im.h});
     *out = A.release();

@@ -17,8 +17,13 @@
 //   HDR   Radiance RGBE, flat and run-length scanlines; floats converted to
 //         8 bits as stbi_load does for HDR files:
 //         (uint8) clamp(pow(v, 1 / 2.2) * 255 + 0.5, 0, 255), alpha 255
+// Files are untrusted: every size comes from the file, so images above
+// kMaxPixels are refused, and no buffer is sized from a header before the
+// data that would fill it is known to be there (inflate output grows with
+// what the stream yields; HDR and JPEG check the bytes left first).
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -47,21 +52,48 @@ struct Fail {
     throw Fail{buf};
 }
 
+// 16384 x 16384: RGBA8 output of at most 1 GiB
+constexpr uint64_t kMaxPixels = 1ull << 28;
+void check_size(const char* fmt, uint64_t w, uint64_t h) {
+    if (w * h > kMaxPixels) fail("%s: %llu x %llu pixels exceed the %llu-pixel limit", fmt, (unsigned long long)w,
+                                 (unsigned long long)h, (unsigned long long)kMaxPixels);
+}
+
 uint32_t be32(const uint8_t* p) { return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]; }
 uint32_t be16(const uint8_t* p) { return ((uint32_t)p[0] << 8) | p[1]; }
 
 // ---- PNG -----------------------------------------------------------------------
+// The first `expect` bytes of the zlib stream `in`. The output buffer grows
+// (doubling, capped at expect) as the stream yields data, so a header that
+// claims a large image costs memory only if the stream delivers it.
 std::vector<uint8_t> inflate_all(const std::vector<uint8_t>& in, size_t expect) {
-    std::vector<uint8_t> out(expect ? expect : 1);
+    if (in.size() > 0xFFFFFFFFu) fail("png: image data too large");
+    std::vector<uint8_t> out(std::min<size_t>(std::max<size_t>(expect, 1), std::max<size_t>(in.size() * 8, 1u << 16)));
     z_stream zs;
     memset(&zs, 0, sizeof(zs));
     if (inflateInit(&zs) != Z_OK) fail("png: zlib init failed");
     zs.next_in = const_cast<Bytef*>(in.data());
     zs.avail_in = (uInt)in.size();
-    zs.next_out = out.data();
-    zs.avail_out = (uInt)out.size();
-    int rc = inflate(&zs, Z_FINISH);
-    size_t got = zs.total_out;
+    int rc = Z_OK;
+    size_t got = 0;
+    for (;;) {
+        const size_t room = std::min<size_t>(out.size() - got, 0x40000000u);
+        zs.next_out = out.data() + got;
+        zs.avail_out = (uInt)room;
+        rc = inflate(&zs, Z_NO_FLUSH);
+        got += room - zs.avail_out;
+        if (rc != Z_OK || got >= expect) break;
+        if (zs.avail_out == 0 && got == out.size()) {  // full: grow towards expect
+            try {
+                out.resize(std::min<size_t>(std::max<size_t>(out.size(), 1) * 2, expect));
+            } catch (...) {
+                inflateEnd(&zs);
+                throw;
+            }
+        } else if (zs.avail_in == 0) {
+            break;  // input exhausted
+        }
+    }
     inflateEnd(&zs);
     if (rc != Z_STREAM_END && rc != Z_BUF_ERROR && rc != Z_OK) fail("png: corrupt image data (zlib %d)", rc);
     if (got < expect) fail("png: image data too short (%zu of %zu bytes)", got, expect);
@@ -205,6 +237,7 @@ void decode_png(const uint8_t* d, size_t n, Image& out) {
             I.interlace = c[12];
             if (c[10] != 0 || c[11] != 0 || I.interlace > 1) fail("png: unsupported compression/filter/interlace");
             if (!I.w || !I.h || I.w > (1u << 24) || I.h > (1u << 24)) fail("png: bad size");
+            check_size("png", I.w, I.h);
             uint32_t ch = channels_of(I.ctype);
             bool ok = I.depth == 8 || I.depth == 16 || ((I.ctype == 0 || I.ctype == 3) && I.depth < 8 &&
                                                         (I.depth == 1 || I.depth == 2 || I.depth == 4));
@@ -242,10 +275,10 @@ void decode_png(const uint8_t* d, size_t n, Image& out) {
     uint32_t bpp = ch * I.depth;
     out.w = I.w;
     out.h = I.h;
-    out.px.assign((size_t)I.w * I.h * 4, 0);
     if (!I.interlace) {
         size_t raw = (size_t)I.h * (((size_t)I.w * bpp + 7) / 8 + 1);
         std::vector<uint8_t> z = inflate_all(idat, raw);
+        out.px.assign((size_t)I.w * I.h * 4, 0);
         std::vector<uint8_t> px = unfilter(z.data(), I.w, I.h, bpp);
         emit(I, px, I.w, I.h, 0, 0, 1, 1, out.px.data());
         return;
@@ -260,6 +293,7 @@ void decode_png(const uint8_t* d, size_t n, Image& out) {
         if (pw[k] && ph[k]) total += (size_t)ph[k] * (((size_t)pw[k] * bpp + 7) / 8 + 1);
     }
     std::vector<uint8_t> z = inflate_all(idat, total);
+    out.px.assign((size_t)I.w * I.h * 4, 0);
     size_t off = 0;
     for (int k = 0; k < 7; ++k) {
         if (!pw[k] || !ph[k]) continue;
@@ -366,6 +400,13 @@ struct Jpeg {
     }
     static int extend(int v, int t) { return t && v < (1 << (t - 1)) ? v - (1 << t) + 1 : v; }
     int receive_extend(int s) { return s ? extend(bits(s), s) : 0; }
+    // a DC difference category (T.81 F.1.2.1: at most 11 for 8-bit samples;
+    // 15 is the most the bit reader takes at once)
+    int dc_category(const Huff& hf) {
+        const int t = decode(hf);
+        if (t > 15) fail("jpeg: bad DC category %d", t);
+        return t;
+    }
     void reset_bits() {
         acc = 0;
         nbits = 0;
@@ -399,10 +440,14 @@ void read_dht(Jpeg& J, int len) {
 }
 
 // progressive and baseline block decoding (ITU T.81 F.2.2, G.1.2)
+// The DC predictor sums the differences modulo 2^32 (a corrupt file may
+// drive it past the int range; valid files stay far inside it).
+inline int add_wrap(int a, int b) { return (int)(int32_t)((uint32_t)a + (uint32_t)b); }
+
 void decode_block_baseline(Jpeg& J, Comp& C, int16_t* blk) {
-    int t = J.decode(J.hdc[C.td]);
+    int t = J.dc_category(J.hdc[C.td]);
     int diff = J.receive_extend(t);
-    C.dc_pred += diff;
+    C.dc_pred = add_wrap(C.dc_pred, diff);
     blk[0] = (int16_t)C.dc_pred;
     for (int k = 1; k < 64;) {
         int rs = J.decode(J.hac[C.ta]);
@@ -421,10 +466,10 @@ void decode_block_baseline(Jpeg& J, Comp& C, int16_t* blk) {
 
 void decode_dc_prog(Jpeg& J, Comp& C, int16_t* blk, int ah, int al) {
     if (ah == 0) {
-        int t = J.decode(J.hdc[C.td]);
+        int t = J.dc_category(J.hdc[C.td]);
         int diff = J.receive_extend(t);
-        C.dc_pred += diff;
-        blk[0] = (int16_t)(C.dc_pred * (1 << al));
+        C.dc_pred = add_wrap(C.dc_pred, diff);
+        blk[0] = (int16_t)((uint32_t)C.dc_pred << al);
     } else if (J.bit()) {
         blk[0] = (int16_t)(blk[0] | (1 << al));
     }
@@ -501,44 +546,49 @@ void decode_ac_refine(Jpeg& J, Comp& C, int16_t* blk, int ss, int se, int al) {
 
 // IJG "islow" integer IDCT with 12-bit constants; input dequantized
 // coefficients in natural order, output 8x8 samples (level shift +128).
-inline int f2f(double x) { return (int)(x * 4096 + 0.5); }
+// The sums and products are taken modulo 2^32 (unsigned), which is what
+// 32-bit int arithmetic gives whenever it does not overflow, and stays
+// defined for the out-of-range coefficients of a corrupt file; the shifts
+// are arithmetic shifts of the 32-bit two's-complement value.
+inline uint32_t f2f(double x) { return (uint32_t)(int)(x * 4096 + 0.5); }
+inline int asr(uint32_t v, int n) { return (int)(int32_t)v >> n; }
 inline uint8_t clamp8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
 
-#define HR_IDCT_1D(s0, s1, s2, s3, s4, s5, s6, s7)          \
-    int t0, t1, t2, t3, p1, p2, p3, p4, p5, x0, x1, x2, x3; \
-    p2 = s2;                                                \
-    p3 = s6;                                                \
-    p1 = (p2 + p3) * f2f(0.5411961);                        \
-    t2 = p1 + p3 * f2f(-1.847759065);                       \
-    t3 = p1 + p2 * f2f(0.765366865);                        \
-    p2 = s0;                                                \
-    p3 = s4;                                                \
-    t0 = (p2 + p3) * 4096;                                  \
-    t1 = (p2 - p3) * 4096;                                  \
-    x0 = t0 + t3;                                           \
-    x3 = t0 - t3;                                           \
-    x1 = t1 + t2;                                           \
-    x2 = t1 - t2;                                           \
-    t0 = s7;                                                \
-    t1 = s5;                                                \
-    t2 = s3;                                                \
-    t3 = s1;                                                \
-    p3 = t0 + t2;                                           \
-    p4 = t1 + t3;                                           \
-    p1 = t0 + t3;                                           \
-    p2 = t1 + t2;                                           \
-    p5 = (p3 + p4) * f2f(1.175875602);                      \
-    t0 = t0 * f2f(0.298631336);                             \
-    t1 = t1 * f2f(2.053119869);                             \
-    t2 = t2 * f2f(3.072711026);                             \
-    t3 = t3 * f2f(1.501321110);                             \
-    p1 = p5 + p1 * f2f(-0.899976223);                       \
-    p2 = p5 + p2 * f2f(-2.562915447);                       \
-    p3 = p3 * f2f(-1.961570560);                            \
-    p4 = p4 * f2f(-0.390180644);                            \
-    t3 += p1 + p4;                                          \
-    t2 += p2 + p3;                                          \
-    t1 += p2 + p4;                                          \
+#define HR_IDCT_1D(s0, s1, s2, s3, s4, s5, s6, s7)               \
+    uint32_t t0, t1, t2, t3, p1, p2, p3, p4, p5, x0, x1, x2, x3; \
+    p2 = (uint32_t)(s2);                                         \
+    p3 = (uint32_t)(s6);                                         \
+    p1 = (p2 + p3) * f2f(0.5411961);                             \
+    t2 = p1 + p3 * f2f(-1.847759065);                            \
+    t3 = p1 + p2 * f2f(0.765366865);                             \
+    p2 = (uint32_t)(s0);                                         \
+    p3 = (uint32_t)(s4);                                         \
+    t0 = (p2 + p3) * 4096u;                                      \
+    t1 = (p2 - p3) * 4096u;                                      \
+    x0 = t0 + t3;                                                \
+    x3 = t0 - t3;                                                \
+    x1 = t1 + t2;                                                \
+    x2 = t1 - t2;                                                \
+    t0 = (uint32_t)(s7);                                         \
+    t1 = (uint32_t)(s5);                                         \
+    t2 = (uint32_t)(s3);                                         \
+    t3 = (uint32_t)(s1);                                         \
+    p3 = t0 + t2;                                                \
+    p4 = t1 + t3;                                                \
+    p1 = t0 + t3;                                                \
+    p2 = t1 + t2;                                                \
+    p5 = (p3 + p4) * f2f(1.175875602);                           \
+    t0 = t0 * f2f(0.298631336);                                  \
+    t1 = t1 * f2f(2.053119869);                                  \
+    t2 = t2 * f2f(3.072711026);                                  \
+    t3 = t3 * f2f(1.501321110);                                  \
+    p1 = p5 + p1 * f2f(-0.899976223);                            \
+    p2 = p5 + p2 * f2f(-2.562915447);                            \
+    p3 = p3 * f2f(-1.961570560);                                 \
+    p4 = p4 * f2f(-0.390180644);                                 \
+    t3 += p1 + p4;                                               \
+    t2 += p2 + p3;                                               \
+    t1 += p2 + p4;                                               \
     t0 += p1 + p3;
 
 void idct_block(uint8_t* out, int out_stride, const int16_t* d) {
@@ -555,32 +605,32 @@ void idct_block(uint8_t* out, int out_stride, const int16_t* d) {
             x1 += 512;
             x2 += 512;
             x3 += 512;
-            o[0] = (x0 + t3) >> 10;
-            o[56] = (x0 - t3) >> 10;
-            o[8] = (x1 + t2) >> 10;
-            o[48] = (x1 - t2) >> 10;
-            o[16] = (x2 + t1) >> 10;
-            o[40] = (x2 - t1) >> 10;
-            o[24] = (x3 + t0) >> 10;
-            o[32] = (x3 - t0) >> 10;
+            o[0] = asr(x0 + t3, 10);
+            o[56] = asr(x0 - t3, 10);
+            o[8] = asr(x1 + t2, 10);
+            o[48] = asr(x1 - t2, 10);
+            o[16] = asr(x2 + t1, 10);
+            o[40] = asr(x2 - t1, 10);
+            o[24] = asr(x3 + t0, 10);
+            o[32] = asr(x3 - t0, 10);
         }
     }
     for (int i = 0; i < 8; ++i) {  // rows
         const int* s = v + 8 * i;
         uint8_t* o = out + (size_t)i * out_stride;
         HR_IDCT_1D(s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7])
-        x0 += 65536 + (128 << 17);
-        x1 += 65536 + (128 << 17);
-        x2 += 65536 + (128 << 17);
-        x3 += 65536 + (128 << 17);
-        o[0] = clamp8((x0 + t3) >> 17);
-        o[7] = clamp8((x0 - t3) >> 17);
-        o[1] = clamp8((x1 + t2) >> 17);
-        o[6] = clamp8((x1 - t2) >> 17);
-        o[2] = clamp8((x2 + t1) >> 17);
-        o[5] = clamp8((x2 - t1) >> 17);
-        o[3] = clamp8((x3 + t0) >> 17);
-        o[4] = clamp8((x3 - t0) >> 17);
+        x0 += 65536u + (128u << 17);
+        x1 += 65536u + (128u << 17);
+        x2 += 65536u + (128u << 17);
+        x3 += 65536u + (128u << 17);
+        o[0] = clamp8(asr(x0 + t3, 17));
+        o[7] = clamp8(asr(x0 - t3, 17));
+        o[1] = clamp8(asr(x1 + t2, 17));
+        o[6] = clamp8(asr(x1 - t2, 17));
+        o[2] = clamp8(asr(x2 + t1, 17));
+        o[5] = clamp8(asr(x2 - t1, 17));
+        o[3] = clamp8(asr(x3 + t0, 17));
+        o[4] = clamp8(asr(x3 - t0, 17));
     }
 }
 #undef HR_IDCT_1D
@@ -591,6 +641,7 @@ void read_sof(Jpeg& J, int len) {
     J.w = J.u16();
     int nc = J.byte();
     if (!J.w || !J.h) fail("jpeg: zero size (DNL is not supported)");
+    check_size("jpeg", (uint64_t)J.w, (uint64_t)J.h);
     if (nc != 1 && nc != 3) fail("jpeg: %d components (1 or 3 supported)", nc);
     if (len != 8 + 3 * nc) fail("jpeg: bad SOF");
     J.c.resize(nc);
@@ -606,6 +657,16 @@ void read_sof(Jpeg& J, int len) {
     }
     J.mcux = (J.w + 8 * J.hmax - 1) / (8 * J.hmax);
     J.mcuy = (J.h + 8 * J.vmax - 1) / (8 * J.vmax);
+    // every block inside the image codes its DC coefficient with at least one
+    // bit in a scan after this frame header
+    uint64_t blocks = 0;
+    for (const auto& C : J.c) {
+        const uint64_t cw = (((uint64_t)J.w * C.h + J.hmax - 1) / J.hmax + 7) / 8;
+        const uint64_t ch = (((uint64_t)J.h * C.v + J.vmax - 1) / J.vmax + 7) / 8;
+        blocks += cw * ch;
+    }
+    if (blocks > 8 * (uint64_t)(J.n - J.p)) fail("jpeg: truncated (%llu blocks in %zu bytes)",
+                                                 (unsigned long long)blocks, J.n - J.p);
     for (auto& C : J.c) {
         C.bw = J.mcux * C.h;
         C.bh = J.mcuy * C.v;
@@ -866,8 +927,28 @@ void decode_hdr(const uint8_t* d, size_t n, Image& out, std::vector<float>* fout
     }
     if (!fmt) fail("hdr: unsupported format");
     std::string res = line();
-    int h = 0, w = 0;
-    if (sscanf(res.c_str(), "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0) fail("hdr: unsupported orientation");
+    // "-Y <height> +X <width>" (the only orientation stbi_load reads)
+    auto field = [&](size_t& k, const char* tag) -> long {
+        const size_t t = strlen(tag);
+        if (res.compare(k, t, tag) != 0) fail("hdr: unsupported orientation");
+        k += t;
+        long v = 0;
+        size_t digits = 0;
+        while (k < res.size() && res[k] >= '0' && res[k] <= '9' && digits < 9) v = v * 10 + (res[k++] - '0'), ++digits;
+        if (!digits || v <= 0) fail("hdr: unsupported orientation");
+        return v;
+    };
+    size_t k = 0;
+    const long hl = field(k, "-Y ");
+    const long wl = field(k, " +X ");
+    if (k != res.size()) fail("hdr: unsupported orientation");
+    check_size("hdr", (uint64_t)wl, (uint64_t)hl);
+    const int h = (int)hl, w = (int)wl;
+    // the least data the scanlines can take: flat RGBE, or per run-length
+    // line a 4-byte header and per channel 2 bytes per 127 values
+    const bool rle_ok = !(w < 8 || w >= 32768);
+    const uint64_t least = rle_ok ? (uint64_t)h * (4u + 8u * (((uint64_t)w + 126u) / 127u)) : (uint64_t)w * h * 4u;
+    if (least > n - p) fail("hdr: truncated data");
     std::vector<uint8_t> rgbe((size_t)w * h * 4);
     auto flat = [&](size_t first_px) {
         size_t need = ((size_t)w * h - first_px) * 4;
@@ -875,7 +956,7 @@ void decode_hdr(const uint8_t* d, size_t n, Image& out, std::vector<float>* fout
         memcpy(&rgbe[first_px * 4], d + p, need);
         p += need;
     };
-    if (w < 8 || w >= 32768) {
+    if (!rle_ok) {
         flat(0);
     } else {
         std::vector<uint8_t> line_buf((size_t)w * 4);
@@ -951,6 +1032,9 @@ bool decode(const uint8_t* data, size_t size, Image& out, std::string& err, std:
         return false;
     } catch (const std::bad_alloc&) {
         err = "out of memory";
+        return false;
+    } catch (const std::exception& e) {
+        err = std::string("decode failed: ") + e.what();
         return false;
     }
     return true;
