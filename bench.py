@@ -195,14 +195,16 @@ def denoise_measure(steps=3, log_fn=log):
     from vulkanhybridraytracer_amd import abi, denoise, hr, scenes
     if not os.path.exists(denoise.WEIGHTS_PATH):
         return {"skipped": "no trained weights at %s" % os.path.relpath(denoise.WEIGHTS_PATH, ROOT)}
-    sc = scenes.make_1080p_4m(samples=4, spp=4, adaptive=False)
+    fr = scenes.config("denoise_1080p_4m")["frame"]
+    lo, hi = fr["low_spp"], fr["high_spp"]
+    sc = scenes.make_1080p_4m(samples=lo, spp=lo, adaptive=fr["adaptive"])
     ctx = hr.Context(0)
     hs = hr.HybridScene(ctx, sc)
     rd = hr.HybridRenderer(ctx, hs)
     rd.gbuffer_primary()
     W, H = sc.width, sc.height
-    low = dict(sc.params, shadow_samples=4, refl_spp=4, flags=0)
-    high = dict(sc.params, shadow_samples=64, refl_spp=64, flags=0)
+    low = dict(sc.params, shadow_samples=lo, refl_spp=lo, flags=0)
+    high = dict(sc.params, shadow_samples=hi, refl_spp=hi, flags=0)
 
     def frame(params, reps):
         rd.set_frame(sc.camera, sc.lights, sc.shapes, params)
@@ -268,9 +270,11 @@ def rt_main(args):
     hit. value = camera + bounce + hit shadow rays per second."""
     from vulkanhybridraytracer_amd import abi, hr, scenes
     sc = scenes.make_1080p_1m()
+    p = scenes.config("rt_1080p_1m")["params"]
     ctx = hr.Context(0)
     rd = hr.HybridRenderer(ctx, hr.HybridScene(ctx, sc))
-    rd.set_frame(sc.camera, sc.lights, sc.shapes, dict(sc.params, shadow_samples=16, refl_spp=1, max_depth=8))
+    rd.set_frame(sc.camera, sc.lights, sc.shapes, dict(sc.params, shadow_samples=p["shadow_samples"],
+                                                       refl_spp=p["refl_spp"], max_depth=p["max_depth"]))
     acc = ctx.alloc(sc.width * sc.height * 16)
     for _ in range(args.warmup):
         rd.render_rt(acc)
@@ -288,7 +292,8 @@ def rt_main(args):
            "unit": "Grays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "rt_1080p_1m", "shadow_samples": 16, "camera_spp": 1, "max_depth": 8},
+           "config": {"workload": "rt_1080p_1m", "shadow_samples": p["shadow_samples"], "camera_spp": p["refl_spp"],
+                      "max_depth": p["max_depth"]},
            "ms_frame": round(float(np.mean(ms)), 3), "rays_per_step": int(rays // max(args.steps, 1))}
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -37,3 +37,55 @@ def test_camera_matches_glm_conventions():
 def test_deterministic():
     a, b = scenes.make_small(seed=4), scenes.make_small(seed=4)
     assert np.array_equal(a.materials, b.materials) and np.array_equal(a.prims["model"], b.prims["model"])
+
+
+WORKLOAD_CONFIGS = ["hybrid_640x360_20k", "hybrid_1080p_1m", "hybrid_1080p_4m", "hybrid_batch32", "rt_1080p_1m",
+                    "denoise_1080p_4m"]
+
+
+def test_every_workload_has_a_config():
+    import glob
+    import json
+    import os
+    names = {os.path.basename(p)[:-5] for p in glob.glob(os.path.join(scenes.CONFIG_DIR, "*.json"))}
+    assert set(WORKLOAD_CONFIGS) | {"terrain_generator"} == names
+    for n in WORKLOAD_CONFIGS:
+        cfg = json.load(open(os.path.join(scenes.CONFIG_DIR, n + ".json")))
+        assert cfg["workload"] == n and cfg["description"]
+
+
+def test_workloads_follow_their_configs():
+    """The generated scenes carry the configs' sizes, camera and sampling."""
+    small = scenes.config("hybrid_640x360_20k")
+    sc = scenes.make_small()
+    cam = small["camera"]
+    assert np.array_equal(sc.camera, scenes.rt_camera(cam["eye"], cam["target"], small["width"], small["height"],
+                                                      fov=cam["fov_deg"], near=cam["near"], far=cam["far"]))
+    assert np.allclose(sc.lights[0, :3], small["lights"][0]["position"])
+    for name, tris in (("hybrid_1080p_1m", 1_048_576), ("hybrid_1080p_4m", 4_194_304)):
+        cfg = scenes.config(name)
+        g = scenes.config("terrain_generator")
+        assert cfg["terrain_quads"] ** 2 * 2 + cfg["instances"] * g["instances"]["triangles"] == tris
+    sc = scenes.make_terrain_scene(quads=16, instances=2, width=32, height=18, samples=64, spp=64, adaptive=True,
+                                   rough_max=1.0)
+    assert sc.params["flags"] & 4 and sc.params["shadow_samples"] == 64
+    b = scenes.config("hybrid_batch32")
+    assert set(map(tuple, b["resolutions"])) == set(scenes.BATCH_RESOLUTIONS)
+
+
+def test_generators_read_the_configs(monkeypatch):
+    """Changing a config value changes the scene: the decisions live in the
+    files, not in copies of them in code."""
+    import copy
+    cfg = copy.deepcopy(scenes.config("hybrid_640x360_20k"))
+    cfg["spheres"]["p_metallic"] = 0.0
+    cfg["camera"]["eye"] = [5.0, 10.0, 40.0]
+    monkeypatch.setitem(scenes._configs, "hybrid_640x360_20k", cfg)
+    sc = scenes.make_small(width=64, height=36, spheres=8)
+    assert np.all(sc.materials[:, 8] == 0.0)
+    assert np.allclose(sc.camera[12:15], (5.0, 10.0, 40.0), atol=1e-5)
+    g = copy.deepcopy(scenes.config("terrain_generator"))
+    g["lights"]["height"] = [70.0, 80.0]
+    monkeypatch.setitem(scenes._configs, "terrain_generator", g)
+    sc = scenes.make_terrain_scene(quads=16, instances=2, soup_tris=16, width=32, height=18)
+    assert np.all((sc.lights[:, 1] >= 70) & (sc.lights[:, 1] <= 80))

@@ -21,12 +21,38 @@ Workloads (sizes in the names):
   hybrid_1080p_4m      the same generator with a 1024x1024-quad terrain and 1024 soups,
                        roughness U[0,1] and adaptive sampling (1-64 shadow rays per
                        light, 1-64 reflection paths per pixel)
+  hybrid_batch32       32 scenes of the terrain generator, 50k-2M triangles, 256x256
+                       to 1920x1080, adaptive sampling up to 16
+
+The sizes, camera, lights, material ranges and sampling of every workload are
+read from configs/<workload>.json at the repository root (the terrain
+generator's constants from configs/terrain_generator.json); the functions'
+keyword arguments override them for tests.
 """
+import json
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import abi
+
+CONFIG_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "configs")
+_configs = {}
+
+
+def config(name):
+    """configs/<name>.json (cached; a workload name or terrain_generator)"""
+    if name not in _configs:
+        with open(os.path.join(CONFIG_DIR, name + ".json")) as f:
+            _configs[name] = json.load(f)
+    return _configs[name]
+
+
+def _params(p):
+    """the trace parameters of a config's "params" block"""
+    return dict(shadow_samples=p["shadow_samples"], refl_spp=p["refl_spp"], max_depth=p["max_depth"],
+                flags=abi.HR_FLAG_ADAPTIVE if p["adaptive"] else 0)
 
 PRIM_DTYPE = np.dtype([("mesh", "<u4"), ("first_index", "<u4"), ("index_count", "<u4"),
                        ("material", "<i4"), ("model", "<f4", 16)])
@@ -94,11 +120,11 @@ def perspective(fovy_deg, aspect, near=0.1, far=1000.0):
     return m
 
 
-def rt_camera(eye, center, width, height, fov=60.0, frame=0.0):
+def rt_camera(eye, center, width, height, fov=60.0, frame=0.0, near=0.1, far=1000.0):
     """RTCameraData as vk_engine.cpp:688-740: inverse(view), inverse(proj) with
     projection[1][1] *= -1, frame."""
     view = look_at(eye, center)
-    proj = perspective(fov, width / height)
+    proj = perspective(fov, width / height, near, far)
     proj[1, 1] *= -1.0
     cam = np.zeros(36, dtype=np.float32)
     cam[0:16] = _colmajor(np.linalg.inv(view))
@@ -250,11 +276,11 @@ def value_noise(x, z, rng, octaves=6, base_cells=4):
     return h / total
 
 
-def terrain_mesh(quads, extent, amplitude, rng):
+def terrain_mesh(quads, extent, amplitude, rng, octaves=6, base_cells=4):
     n = quads + 1
     t = np.linspace(0.0, 1.0, n)
     X, Z = np.meshgrid(t, t, indexing="ij")
-    Hh = amplitude * value_noise(X, Z, rng)
+    Hh = amplitude * value_noise(X, Z, rng, octaves, base_cells)
     px = (X - 0.5) * extent
     pz = (Z - 0.5) * extent
     step = extent / quads
@@ -306,110 +332,151 @@ def _prims(rows):
 
 
 # ---------------------------------------------------------------- workloads
-def make_small(seed=0, width=640, height=360, spheres=64, frame=0.0):
-    """hybrid_640x360_20k: 2-triangle 100x100 ground + 64 subdivision-2 icospheres
-    (20,482 triangles), centres U([-40,40]x[1,10]x[-40,40]), radii U[0.5,3];
-    albedo U[0,1]^3, metallic 1 (shading model 3) with p = 0.3 else 0 (model 0),
-    roughness 0; one point light at (0,50,0) (radius 0: hard shadows) and one
-    directional light; 1 shadow ray per light, mirror chains up to depth 8."""
+def _light(d):
+    """uboLight row of a config's light entry"""
+    if d["type"] == "directional":
+        return directional_light(d["direction"], intensity=d["intensity"], radius=d["radius"])
+    return point_light(d["position"], intensity=d["intensity"], max_distance=d["max_distance"], radius=d["radius"])
+
+
+def make_small(seed=None, width=None, height=None, spheres=None, frame=0.0):
+    """hybrid_640x360_20k (configs/hybrid_640x360_20k.json): 2-triangle 100x100
+    ground + 64 subdivision-2 icospheres (20,482 triangles), centres
+    U([-40,40]x[1,10]x[-40,40]), radii U[0.5,3]; albedo U[0,1]^3, metallic 1
+    (shading model 3) with p = 0.3 else 0 (model 0), roughness 0; one point
+    light at (0,50,0) (radius 0: hard shadows) and one directional light; 1
+    shadow ray per light, mirror chains up to depth 8."""
+    cfg = config("hybrid_640x360_20k")
+    seed = cfg["seed"] if seed is None else seed
+    width = cfg["width"] if width is None else width
+    height = cfg["height"] if height is None else height
+    sp = cfg["spheres"]
+    spheres = sp["count"] if spheres is None else spheres
     rng = np.random.default_rng(seed)
-    ground = quad_mesh(100.0)
-    sphere = icosphere(2)
+    ground = quad_mesh(cfg["ground"]["size"])
+    sphere = icosphere(sp["subdivision"])
     mats, rows = [], []
     n_obj = 1 + spheres
     for i in range(n_obj):
-        metallic = 1.0 if rng.uniform() < 0.3 else 0.0
+        metallic = 1.0 if rng.uniform() < sp["p_metallic"] else 0.0
         albedo = rng.uniform(0, 1, size=3)
         mats.append(material(i, albedo, 3 if metallic else 0, metallic, 0.0))
     rows.append((0, 0, len(ground[1]), 0, np.eye(4)))
     for i in range(spheres):
-        c = rng.uniform([-40, 1, -40], [40, 10, 40])
-        r = rng.uniform(0.5, 3.0)
+        c = rng.uniform(sp["centre_min"], sp["centre_max"])
+        r = rng.uniform(*sp["radius"])
         rows.append((1, 0, len(sphere[1]), 1 + i, translate_scale(c, r)))
-    lights = np.stack([point_light((0, 50, 0), intensity=1000.0, max_distance=500.0, radius=0.0),
-                       directional_light((0.4, 1.0, 0.3), intensity=1.0)])
-    sky, irr = make_env()
-    return Scene("hybrid_640x360_20k", [ground, sphere], _prims(rows), np.stack(mats), lights, None,
-                 rt_camera((0, 25, 70), (0, 0, 0), width, height, frame=frame), sky, irr, width, height,
-                 dict(shadow_samples=1, refl_spp=1, max_depth=8))
+    lights = np.stack([_light(d) for d in cfg["lights"]])
+    sky, irr = make_env(*cfg["env"]["sky"], *cfg["env"]["irradiance"])
+    cam = cfg["camera"]
+    return Scene(cfg["workload"], [ground, sphere], _prims(rows), np.stack(mats), lights, None,
+                 rt_camera(cam["eye"], cam["target"], width, height, fov=cam["fov_deg"], frame=frame,
+                           near=cam["near"], far=cam["far"]), sky, irr, width, height,
+                 {k: v for k, v in _params(cfg["params"]).items() if k != "flags"})
 
 
-def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256, soup_tris=2048,
+def make_terrain_scene(seed=0, width=1920, height=1080, quads=512, instances=256, soup_tris=None,
                        extent=200.0, samples=16, spp=4, max_depth=1, frame=0.0, name=None, rough_max=0.6,
                        adaptive=False):
-    """hybrid_1080p_1m (quads=512, instances=256) / hybrid_1080p_4m (1024, 1024):
+    """The terrain generator (configs/terrain_generator.json) of
+    hybrid_1080p_1m (quads=512, instances=256) / hybrid_1080p_4m (1024, 1024):
     value-noise terrain (6 octaves, amplitude 20) with a glossy material
     (shading model 3, metallic 1, roughness U[0,0.6]); triangle-soup instances
     with uniform positions, uniformly random rotations and scales U[1,5],
     materials glossy (model 3) with p = 0.5 else PBR (model 0), roughness
     U[0,0.6]; four 4x4 rectangular area lights at heights U[30,60]."""
+    g = config("terrain_generator")
+    gt, gi, gl, gc = g["terrain"], g["instances"], g["lights"], g["camera"]
+    soup_tris = gi["triangles"] if soup_tris is None else soup_tris
     rng = np.random.default_rng(seed)
-    tV, tI, height_at = terrain_mesh(quads, extent, 20.0, rng)
+    tV, tI, height_at = terrain_mesh(quads, extent, gt["amplitude"], rng, gt["octaves"], gt["base_cells"])
     meshes = [(tV, tI)]
-    mats = [material(0, rng.uniform(0.3, 0.9, size=3), 3, 1.0, rng.uniform(0.0, rough_max))]
+    mats = [material(0, rng.uniform(*gt["albedo"], size=3), gt["shading_model"], gt["metallic"],
+                     rng.uniform(0.0, rough_max))]
     rows = [(0, 0, len(tI), 0, np.eye(4))]
-    half = extent * 0.45
+    half = extent * gi["placement_frac"]
     for i in range(instances):
-        meshes.append(soup_mesh(soup_tris, rng))
-        glossy = rng.uniform() < 0.5
+        meshes.append(soup_mesh(soup_tris, rng, gi["triangle_size"]))
+        glossy = rng.uniform() < gi["p_glossy"]
         mats.append(material(1 + i, rng.uniform(0, 1, size=3), 3 if glossy else 0,
                              1.0 if glossy else 0.0, rng.uniform(0.0, rough_max)))
-        s = rng.uniform(1.0, 5.0)
+        s = rng.uniform(*gi["scale"])
         x, z = rng.uniform(-half, half, size=2)
-        y = height_at(x, z) + 0.5 * s + 0.5
+        y = height_at(x, z) + 0.5 * s + gi["lift"]
         m = translate_scale((x, y, z), s)
         m[:3, :3] = random_rotation(rng) * s
         rows.append((1 + i, 0, soup_tris * 3, 1 + i, m))
     lights, shapes = [], []
-    for _ in range(4):
-        x, z = rng.uniform(-0.3 * extent, 0.3 * extent, size=2)
-        y = rng.uniform(30.0, 60.0)
-        lights.append(point_light((x, y, z), color=rng.uniform(0.7, 1.0, size=3), intensity=3000.0,
-                                  max_distance=500.0, radius=2.0))
-        shapes.append(np.array([1, 4, 0, 0, 0, 0, 4, 0], dtype=np.float32))
-    sky, irr = make_env()
-    cam = rt_camera((0, 0.3 * extent, 0.62 * extent), (0, 0, -0.05 * extent), width, height, frame=frame)
+    assert gl["type"] == "rect"
+    for _ in range(gl["count"]):
+        x, z = rng.uniform(-gl["xz_frac"] * extent, gl["xz_frac"] * extent, size=2)
+        y = rng.uniform(*gl["height"])
+        lights.append(point_light((x, y, z), color=rng.uniform(*gl["color"], size=3), intensity=gl["intensity"],
+                                  max_distance=gl["max_distance"], radius=gl["radius"]))
+        su, sv = gl["size"]
+        shapes.append(np.array([1, su, 0, 0, 0, 0, sv, 0], dtype=np.float32))
+    sky, irr = make_env(*g["env"]["sky"], *g["env"]["irradiance"])
+    cam = rt_camera(tuple(f * extent for f in gc["eye_frac"]), tuple(f * extent for f in gc["target_frac"]),
+                    width, height, fov=gc["fov_deg"], frame=frame, near=gc["near"], far=gc["far"])
     nm = name or ("hybrid_%dx%d_%dtri" % (width, height, quads * quads * 2 + instances * soup_tris))
     return Scene(nm, meshes, _prims(rows), np.stack(mats), np.stack(lights), np.stack(shapes), cam, sky,
                  irr, width, height, dict(shadow_samples=samples, refl_spp=spp, max_depth=max_depth,
                                           flags=abi.HR_FLAG_ADAPTIVE if adaptive else 0))
 
 
-def make_1080p_1m(seed=0, **kw):
-    return make_terrain_scene(seed, quads=512, instances=256, extent=200.0, name="hybrid_1080p_1m", **kw)
+def _terrain_workload(name, seed, kw):
+    cfg = config(name)
+    p = cfg["params"]
+    kw.setdefault("width", cfg["width"])
+    kw.setdefault("height", cfg["height"])
+    kw.setdefault("samples", p["shadow_samples"])
+    kw.setdefault("spp", p["refl_spp"])
+    kw.setdefault("max_depth", p["max_depth"])
+    kw.setdefault("adaptive", p["adaptive"])
+    kw.setdefault("rough_max", cfg["roughness"][1])
+    return make_terrain_scene(cfg["seed"] if seed is None else seed, quads=cfg["terrain_quads"],
+                              instances=cfg["instances"], extent=cfg["extent"], name=cfg["workload"], **kw)
 
 
-def make_1080p_4m(seed=0, **kw):
-    """The 4.2M-triangle scene with adaptive sampling: shadows 1 / 4 / 64 rays
-    per light by penumbra probes, reflections 1 + floor(63 * roughness) paths,
-    roughness U[0,1]."""
-    kw.setdefault("samples", 64)
-    kw.setdefault("spp", 64)
-    kw.setdefault("rough_max", 1.0)
-    kw.setdefault("adaptive", True)
-    return make_terrain_scene(seed, quads=1024, instances=1024, extent=400.0, name="hybrid_1080p_4m", **kw)
+def make_1080p_1m(seed=None, **kw):
+    """hybrid_1080p_1m (configs/hybrid_1080p_1m.json)"""
+    return _terrain_workload("hybrid_1080p_1m", seed, kw)
 
 
-BATCH_RESOLUTIONS = ((256, 256), (640, 360), (1280, 720), (1920, 1080))
+def make_1080p_4m(seed=None, **kw):
+    """hybrid_1080p_4m (configs/hybrid_1080p_4m.json): the 4.2M-triangle scene
+    with adaptive sampling: shadows 1 ray per hard light, else stages of 4,
+    16, 32, 64 rays per light until the penumbra estimate settles;
+    reflections 1 + floor(63 * roughness) paths, roughness U[0,1]."""
+    return _terrain_workload("hybrid_1080p_4m", seed, kw)
 
 
-def make_batch(count=32, seed_base=0, tri_range=(50_000, 2_000_000), resolutions=BATCH_RESOLUTIONS, spp=16):
-    """hybrid_batch32: `count` scenes of the terrain generator with seeds
-    seed_base.., triangle counts log-uniform in tri_range (half terrain, half
-    2048-triangle soups), resolutions drawn from `resolutions`, adaptive
-    sampling up to `spp` (shadow penumbra samples and glossy paths),
-    roughness U[0,1]."""
+BATCH_RESOLUTIONS = tuple(tuple(r) for r in config("hybrid_batch32")["resolutions"])
+
+
+def make_batch(count=None, seed_base=None, tri_range=None, resolutions=BATCH_RESOLUTIONS, spp=None):
+    """hybrid_batch32 (configs/hybrid_batch32.json): `count` scenes of the
+    terrain generator with seeds seed_base.., triangle counts log-uniform in
+    tri_range (half terrain, half 2048-triangle soups), resolutions drawn from
+    `resolutions`, adaptive sampling up to `spp` (shadow penumbra samples and
+    glossy paths), roughness U[0,1]."""
+    cfg = config("hybrid_batch32")
+    count = cfg["count"] if count is None else count
+    seed_base = cfg["seeds"][0] if seed_base is None else seed_base
+    tri_range = cfg["triangles_log_uniform"] if tri_range is None else tri_range
+    spp = cfg["params"]["shadow_samples"] if spp is None else spp
+    soup = config("terrain_generator")["instances"]["triangles"]
     out = []
     for k in range(count):
-        rng = np.random.default_rng(10_000 + seed_base + k)
+        rng = np.random.default_rng(cfg["rng_seed_base"] + seed_base + k)
         tris = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
         quads = max(8, int(round(np.sqrt(tris / 4.0))))
-        instances = max(1, int(round(tris / 2.0 / 2048.0)))
+        instances = max(1, int(round(tris / 2.0 / soup)))
         w, h = resolutions[int(rng.integers(len(resolutions)))]
         extent = 200.0 * quads / 512.0 + 40.0
         out.append(make_terrain_scene(seed=seed_base + k, width=w, height=h, quads=quads, instances=instances,
-                                      extent=extent, samples=spp, spp=spp, rough_max=1.0, adaptive=True,
-                                      name="batch%02d" % k))
+                                      extent=extent, samples=spp, spp=spp, rough_max=cfg["roughness"][1],
+                                      adaptive=cfg["params"]["adaptive"], name="batch%02d" % k))
     return out
 
 
