@@ -596,6 +596,16 @@ void hr_denoiser_destroy(hr_denoiser* dn);
 hr_status hr_denoiser_get_weights(hr_denoiser* dn, float* weights, uint64_t count);
 /* Replaces the weights and resets the optimiser state. */
 hr_status hr_denoiser_set_weights(hr_denoiser* dn, const float* weights, uint64_t count);
+/* Training state for a resumable checkpoint: the FP32 master weights, Adam's
+ * first and second moments (each `count` values in the weight exchange
+ * layout) and the number of Adam steps taken, which the bias corrections
+ * continue from. Training resumed from a state gives the weights that
+ * uninterrupted training gives, bit for bit. adam_m / adam_v / step may be
+ * NULL in get_state; set_state needs all three arrays. */
+hr_status hr_denoiser_get_state(hr_denoiser* dn, float* weights, float* adam_m, float* adam_v, uint64_t count,
+                                uint64_t* step);
+hr_status hr_denoiser_set_state(hr_denoiser* dn, const float* weights, const float* adam_m, const float* adam_v,
+                                uint64_t count, uint64_t step);
 
 /* One image: its G-buffer (position, normal, albedo and material planes are
  * read), the noisy lit colour, and for training the reference colour. All

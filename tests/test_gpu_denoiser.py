@@ -207,3 +207,45 @@ def test_weights_roundtrip_and_errors(gpu_ctx, tmp_path):
         dn.train_step([], 1e-3)
     dn.close()
     dn2.close()
+
+
+def test_checkpoint_resume_is_exact(gpu_ctx, tmp_path):
+    """Training interrupted by a checkpoint (FP32 master weights, Adam m and v,
+    step count, loss kind) and resumed in a fresh denoiser ends with the
+    weights and optimiser state of uninterrupted training, bit for bit."""
+    from vulkanhybridraytracer_amd import denoise, hr
+    frames, host, imgs = _random_frames(gpu_ctx, 17, 2, 32, 32)
+    lrs = [1e-3, 8e-4, 6e-4, 5e-4, 4e-4, 3e-4]
+    straight = denoise.Denoiser(gpu_ctx, seed=3)
+    straight.set_loss(straight.LOSS_REL_MSE)
+    for lr in lrs:
+        straight.train_step(imgs, lr, want_loss=False)
+    want = straight.get_state()
+    assert want[3] == len(lrs) and np.abs(want[1]).max() > 0 and want[2].max() > 0
+    first = denoise.Denoiser(gpu_ctx, seed=3)
+    first.set_loss(first.LOSS_REL_MSE)
+    for lr in lrs[:3]:
+        first.train_step(imgs, lr, want_loss=False)
+    p = str(tmp_path / "ckpt.npz")
+    first.save_checkpoint(p, {"note": "half way"})
+    first.close()
+    resumed, meta = denoise.Denoiser.load_checkpoint(gpu_ctx, p)
+    assert meta == {"note": "half way"} and resumed.loss == resumed.LOSS_REL_MSE
+    for lr in lrs[3:]:
+        resumed.train_step(imgs, lr, want_loss=False)
+    got = resumed.get_state()
+    assert got[3] == want[3]
+    for a, b in zip(got[:3], want[:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    # replacing the weights starts the optimiser afresh
+    resumed.set_weights(got[0])
+    w, m, v, step = resumed.get_state()
+    assert step == 0 and not m.any() and not v.any() and np.array_equal(w, got[0])
+    bad = got[2].copy()
+    bad[5] = -1.0
+    with pytest.raises(hr.HRError):
+        resumed.set_state(got[0], got[1], bad, 6)
+    with pytest.raises(ValueError):
+        resumed.set_state(got[0][:-1], got[1], got[2], 6)
+    straight.close()
+    resumed.close()

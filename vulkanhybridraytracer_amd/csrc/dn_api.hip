@@ -586,6 +586,51 @@ hr_status hr_denoiser_set_weights(hr_denoiser* dn, const float* weights, uint64_
     return upload_weights(dn, weights);
 }
 
+hr_status hr_denoiser_get_state(hr_denoiser* dn, float* weights, float* adam_m, float* adam_v, uint64_t count,
+                                uint64_t* step) {
+    if (!dn) return HR_E_INVALID;
+    hr_context* ctx = dn->ctx;
+    if (!weights || count != public_count()) return fail(ctx, HR_E_INVALID, "state: %llu values expected",
+                                                         (unsigned long long)public_count());
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float> in(dn->nparam);
+    float* dst[3] = {weights, adam_m, adam_v};
+    const float* src[3] = {dn->w, dn->m, dn->v};
+    for (int k = 0; k < 3; ++k) {
+        if (!dst[k]) continue;
+        HR_HIP(ctx, hipMemcpy(in.data(), src[k], dn->nparam * sizeof(float), hipMemcpyDeviceToHost));
+        internal_to_public(dn, in.data(), dst[k]);
+    }
+    if (step) *step = dn->step;
+    return HR_OK;
+}
+
+hr_status hr_denoiser_set_state(hr_denoiser* dn, const float* weights, const float* adam_m, const float* adam_v,
+                                uint64_t count, uint64_t step) {
+    if (!dn) return HR_E_INVALID;
+    hr_context* ctx = dn->ctx;
+    if (!weights || !adam_m || !adam_v || count != public_count())
+        return fail(ctx, HR_E_INVALID, "state: weights, adam_m and adam_v of %llu values expected",
+                    (unsigned long long)public_count());
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(weights[i]) || !std::isfinite(adam_m[i]) || !(adam_v[i] >= 0.0f) ||
+            !std::isfinite(adam_v[i]))
+            return fail(ctx, HR_E_INVALID, "state: value %llu is not finite (or a negative second moment)",
+                        (unsigned long long)i);
+    HR_HIP(ctx, hipSetDevice(ctx->device));
+    HR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float> in;
+    float* dst[3] = {dn->w, dn->m, dn->v};
+    const float* src[3] = {weights, adam_m, adam_v};
+    for (int k = 0; k < 3; ++k) {
+        public_to_internal(dn, src[k], in);
+        HR_HIP(ctx, hipMemcpy(dst[k], in.data(), dn->nparam * sizeof(float), hipMemcpyHostToDevice));
+    }
+    dn->step = step;
+    return pack_all(dn);
+}
+
 hr_status hr_denoiser_set_loss(hr_denoiser* dn, uint32_t loss) {
     if (!dn) return HR_E_INVALID;
     if (loss != HR_DN_LOSS_L1_TONEMAPPED && loss != HR_DN_LOSS_REL_MSE)

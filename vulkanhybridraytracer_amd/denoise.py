@@ -5,13 +5,15 @@ The denoiser has no counterpart in the reference, whose only "denoise" is
 the temporal shadow filter (blur.comp:35-81). It is a kernel-predicting
 U-Net run entirely by the library's HIP kernels (BF16 MFMA convolutions):
   Denoiser        weights, inference (hr_denoise), training steps and
-                  gradients (hr_denoiser_train_step / _gradients)
+                  gradients (hr_denoiser_train_step / _gradients), resumable
+                  checkpoints (hr_denoiser_get_state / _set_state)
   PairRenderer    renders training / evaluation pairs with the tracer: the
                   same view at low (4 spp) and high (64 spp) sample counts,
                   buffers kept in device memory
   conv3x3_forward / conv3x3_backward   one layer, for kernel-level tests
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -88,6 +90,42 @@ class Denoiser:
         w = np.ascontiguousarray(w, dtype=np.float32)
         self.ctx.check(self.ctx.L.hr_denoiser_set_weights(self.h, w.ctypes.data, w.size))
 
+    def get_state(self):
+        """(weights, adam_m, adam_v, step): the FP32 training state"""
+        w, m, v = (np.empty(self.count, dtype=np.float32) for _ in range(3))
+        step = C.c_uint64(0)
+        self.ctx.check(self.ctx.L.hr_denoiser_get_state(self.h, w.ctypes.data, m.ctypes.data, v.ctypes.data,
+                                                        self.count, C.byref(step)))
+        return w, m, v, step.value
+
+    def set_state(self, weights, adam_m, adam_v, step):
+        w, m, v = (np.ascontiguousarray(a, dtype=np.float32) for a in (weights, adam_m, adam_v))
+        for a in (w, m, v):
+            if a.size != self.count:
+                raise ValueError("expected %d values, got %d" % (self.count, a.size))
+        self.ctx.check(self.ctx.L.hr_denoiser_set_state(self.h, w.ctypes.data, m.ctypes.data, v.ctypes.data,
+                                                        self.count, int(step)))
+
+    def save_checkpoint(self, path, meta=None):
+        """Resumable training state as .npz (arrays only, no pickles): FP32
+        master weights, Adam moments, step count, loss kind, and `meta` (a
+        JSON-serialisable dict, e.g. the batch sampler's state)."""
+        w, m, v, step = self.get_state()
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = path + ".tmp.npz"
+        np.savez(tmp, weights=w, adam_m=m, adam_v=v, step=np.uint64(step), loss=np.uint32(self.loss),
+                 meta=np.array(json.dumps(meta or {})))
+        os.replace(tmp, path)
+
+    @classmethod
+    def load_checkpoint(cls, ctx, path):
+        """(Denoiser, meta) restored from save_checkpoint's file"""
+        with np.load(path, allow_pickle=False) as z:
+            dn = cls(ctx)
+            dn.set_loss(int(z["loss"]))
+            dn.set_state(z["weights"], z["adam_m"], z["adam_v"], int(z["step"]))
+            return dn, json.loads(str(z["meta"]))
+
     def save(self, path):
         """BF16 bit patterns (the precision the convolutions use)."""
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
@@ -115,8 +153,11 @@ class Denoiser:
 
     LOSS_L1_TONEMAPPED, LOSS_REL_MSE = 0, 1
 
+    loss = LOSS_L1_TONEMAPPED
+
     def set_loss(self, kind):
         self.ctx.check(self.ctx.L.hr_denoiser_set_loss(self.h, kind))
+        self.loss = kind
 
     def train_step(self, images, lr=1e-3, want_loss=True):
         """One Adam step; without want_loss nothing waits for the device."""
@@ -225,11 +266,17 @@ class PairRenderer:
         self.hs.close()
 
 
-def training_scene(seed, quads=256, instances=128, extent=120.0):
+def training_scene(seed, quads=None, instances=None, extent=None):
     """A smaller scene of the terrain generator (393,216 triangles, roughness
-    U[0,1]); its views are 256 x 256."""
-    return scenes.make_terrain_scene(seed=seed, width=256, height=256, quads=quads, instances=instances,
-                                     extent=extent, rough_max=1.0, name="train%04d" % seed)
+    U[0,1]; configs/denoise_1080p_4m.json "training"); its views are 256 x 256."""
+    t = scenes.config("denoise_1080p_4m")["training"]
+    g = t["scene_generator"]
+    w, h = t["crop"]
+    return scenes.make_terrain_scene(seed=seed, width=w, height=h,
+                                     quads=g["terrain_quads"] if quads is None else quads,
+                                     instances=g["instances"] if instances is None else instances,
+                                     extent=g["extent"] if extent is None else extent,
+                                     rough_max=g["roughness"][1], name="train%04d" % seed)
 
 
 def make_dataset(ctx, seeds, views, size=256, low_spp=4, high_spp=64, log=None, scene_kw=None):

@@ -119,6 +119,8 @@ def load(path=LIB_PATH):
         "hr_denoiser_destroy": (None, [vp]),
         "hr_denoiser_get_weights": (st, [vp, vp, u64]),
         "hr_denoiser_set_weights": (st, [vp, vp, u64]),
+        "hr_denoiser_get_state": (st, [vp, vp, vp, vp, u64, C.POINTER(u64)]),
+        "hr_denoiser_set_state": (st, [vp, vp, vp, vp, u64, u64]),
         "hr_denoiser_set_loss": (st, [vp, u32]),
         "hr_denoise": (st, [vp, C.POINTER(abi.hr_dn_image), C.POINTER(C.c_float)]),
         "hr_denoiser_train_step": (st, [vp, vp, u32, C.c_float, C.POINTER(C.c_float)]),
