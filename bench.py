@@ -348,7 +348,9 @@ def main():
         ctx.set_tuning(t["blocks_per_cu"], t["tile_w"], args.wave_budget_mb)
     hs = hr.HybridScene(ctx, sc, leaf_max=args.leaf_max)
     info = hs.info()
-    log("LBVH build %.2f ms, depth %d" % (info["ms_build"], info["depth"]))
+    mem = hs.memory()
+    log("LBVH build %.2f ms, depth %d, %d binary / %d 4-wide nodes, scene %.1f MiB on the device" % (
+        info["ms_build"], info["depth"], info["node_count"], mem["wide_count"], mem["device_bytes"] / 2**20))
     rd = hr.HybridRenderer(ctx, hs)
     gst = rd.rasterize_hybrid(stats=True)  # G-buffer + one pass (also warms up)
     log("first pass", gst)
@@ -431,7 +433,9 @@ def main():
            "rays_per_step": per_step, "ms_shadow": round(avg_shadow_ms, 3),
            "ms_shade": round(float(np.mean(ms_shade)), 3), "ms_gbuffer": round(gst["ms_gbuffer"], 3),
            "ms_lbvh_build": round(info["ms_build"], 3), "roofline": roofline, "cpu_baseline": cpu,
-           "parity": parity}
+           "parity": parity,
+           "scene_memory": {"binary_nodes": info["node_count"], "wide_nodes": mem["wide_count"],
+                            "device_bytes": mem["device_bytes"]}}
     # the closest-hit queue kernel of the reflection stage, from its own profile
     # (it runs inside hr_shade_hybrid, so its launch time is rocprofv3's)
     prof_r, src_r = load_profile("k_wf_trace", sc.name)

@@ -310,15 +310,23 @@ void hr_scene_destroy(hr_scene* scene);
 /* Scene layout queries and device-array export (for validation). */
 hr_status hr_scene_info(const hr_scene* scene, uint32_t* tri_count, uint32_t* node_count,
                         uint32_t* depth, float* ms_build);
+/* The number of 4-wide traversal nodes and the device memory the scene holds
+ * (geometry, both node arrays, build scratch kept for refits, textures,
+ * environment). No counterpart in the reference, whose acceleration
+ * structures live in Vulkan memory the driver sizes (renderer.cpp:1643-1916). */
+hr_status hr_scene_memory(const hr_scene* scene, uint32_t* wide_count, uint64_t* device_bytes);
 /* which: 0 = traversal nodes (node_count x 64 B), 1 = sorted world-space
  * triangles (tri_count x 48 B), 2 = Morton codes in sorted order (tri_count x
  * 4 B), 3 = sorted triangle ids (tri_count x 4 B), 4 = Karras internal nodes
  * (left, right, first, last) ((tri_count - 1) x 16 B), 5 = 4-wide traversal
- * nodes (node_count x 128 B; the slot of each wide root, a binary node, holds
- * its 2-4 children: from the root down, a wide node takes its binary node's
- * two children and twice replaces the internal child of largest box area,
- * (x y + y z) + z x of the extents, first on ties, by that child's two
- * children). */
+ * nodes (wide_count x 128 B of hr_scene_memory, the root first: from the root
+ * down, a wide node takes its binary node's two children and twice replaces
+ * the internal child of largest box area, (x y + y z) + z x of the extents,
+ * first on ties, by that child's two children; the wide nodes are numbered
+ * densely in the order of the binary nodes they are rooted at, internal child
+ * codes are those numbers, and word 28 of a node holds its binary node).
+ * The wide topology is chosen at build time; a refit keeps it and updates
+ * the boxes. */
 hr_status hr_scene_export(const hr_scene* scene, uint32_t which, void* dst_host, size_t bytes);
 
 /* ---- per-frame passes ---------------------------------------------------- */

@@ -237,22 +237,47 @@ def _binary_children(nodes, i):
 
 @pytest.mark.parametrize("leaf_max", [1, 4])
 def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
-    """the 4-wide nodes of the timed kernels: from the root they reach exactly
-    the binary tree's leaves, each child box equal to the binary box of the
-    same subtree, empty slots with inverted boxes, and every wide node is the
-    greedy largest-area collapse of its binary node"""
+    """the 4-wide nodes of the timed kernels: stored densely (the root first,
+    then in the order of the binary nodes they are rooted at, about a third
+    as many as binary nodes); from the root they reach exactly the binary
+    tree's leaves, each child box equal to the binary box of the same subtree,
+    empty slots with inverted boxes, and every wide node is the greedy
+    largest-area collapse of its binary node. A refit keeps the wide topology
+    and the boxes follow the refitted binary nodes."""
     sc = scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=24, soup_tris=256)
     hr = _hr()
     hs = hr.HybridScene(gpu_ctx, sc, leaf_max=leaf_max)
+    LEAF = 0x80000000
+    check_wide_nodes(hs, LEAF, check_collapse=True)
+    # refit with moved instances: same wide topology, boxes of the new binary nodes
+    w_before = hs.export(5).reshape(-1, 32)
+    models = sc.prims["model"].copy()
+    rng = np.random.default_rng(5)
+    models[1:, 12:15] += rng.uniform(-3, 3, size=(len(models) - 1, 3)).astype(np.float32)
+    hs.refit(models)
+    w_after = hs.export(5).reshape(-1, 32)
+    assert np.array_equal(w_before[:, 24:29], w_after[:, 24:29])  # codes and binary roots kept
+    assert not np.array_equal(w_before[:, :24], w_after[:, :24])
+    check_wide_nodes(hs, LEAF, check_collapse=False)
+
+
+def check_wide_nodes(hs, LEAF, check_collapse):
     bn = hs.export(0).view(np.float32).reshape(-1, 16)
     w4 = hs.export(5).reshape(-1, 32)
-    LEAF = 0x80000000
+    mem = hs.memory()
+    assert mem["wide_count"] == len(w4) and 0 < len(w4) < 0.5 * len(bn)
+    binary_of = w4[:, 28].astype(np.int64)
+    assert binary_of[0] == 0 and np.all(np.diff(binary_of) > 0)
+    dense_of = {int(b): d for d, b in enumerate(binary_of)}
 
-    def wide(i):
-        f = w4[i].view(np.float32)
+    def wide(i):  # by binary root, internal codes mapped back to binary ids
+        row = w4[dense_of[i]]
+        f = row.view(np.float32)
         lo = np.stack([f[0:4], f[8:12], f[16:20]], -1)
         hi = np.stack([f[4:8], f[12:16], f[20:24]], -1)
-        return [(int(w4[i][24 + k]), lo[k], hi[k]) for k in range(4)]
+        codes = [int(row[24 + k]) for k in range(4)]
+        codes = [c if c == 0xFFFFFFFF or c & LEAF else int(binary_of[c]) for c in codes]
+        return [(codes[k], lo[k], hi[k]) for k in range(4)]
 
     bin_leaves, wide_leaves = [], []
     stack = [0]
@@ -281,6 +306,8 @@ def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
                 stack.append(code)
     assert sorted(bin_leaves) == sorted(wide_leaves)
     assert len(wide_leaves) > 100
+    if not check_collapse:
+        return
 
     # the collapse rule itself: from the binary node's two children, twice open
     # the internal child of largest area (x y + y z) + z x of the extents in
@@ -315,4 +342,4 @@ def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
             else:
                 assert got[k][0] == 0xFFFFFFFF, (i, k)
         stack += [c for c, _, _ in ch if not c & LEAF]
-    assert roots > 10
+    assert roots > 10 and roots == len(w4)

@@ -23,6 +23,7 @@ struct hr_scene {
     std::vector<float> h_models;
     std::vector<uint32_t> h_prim_tri_offset;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // size of each allocs entry
     uint8_t* d_verts = nullptr;
     float4* d_vnormal = nullptr;
     uint32_t* d_indices = nullptr;
@@ -54,6 +55,7 @@ hr_status scene_alloc(hr_scene* s, T** p, size_t count) {
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return hip_fail(s->ctx, e, "hipMalloc");
     s->allocs.push_back(q);
+    s->alloc_bytes.push_back(bytes);
     *p = static_cast<T*>(q);
     return HR_OK;
 }
@@ -64,6 +66,7 @@ void scene_free(hr_scene* s, void* p) {
         if (s->allocs[i] == p) {
             (void)hipFree(p);
             s->allocs.erase(s->allocs.begin() + (long)i);
+            s->alloc_bytes.erase(s->alloc_bytes.begin() + (long)i);
             return;
         }
     }
@@ -108,6 +111,7 @@ DevScene dev_scene(const hr_scene* s) {
     d.sky = s->d_sky;
     d.irr = s->d_irr;
     d.node_count = s->node_count;
+    d.wide_count = s->bb.wide_count;
     d.tri_count = s->ntri;
     d.nmat = s->nmat;
     d.prim_count = s->prim_count;
@@ -437,12 +441,12 @@ SplPlan plan_spl(const FramePlan* plans, size_t nplans, uint32_t ngroups) {
 
 // Kernels of a shadow pass (device jobs already uploaded); scratch holds
 // sp.scratch bytes when sp.use.
-// (node_count: the 4-wide node slots of the first frame's scene; used for one frame)
+// (wide_count: the 4-wide nodes of the first frame's scene; used for one frame)
 hr_status enqueue_shadows(hr_context* ctx, const ShadowJob* d_jobs, uint32_t n, uint32_t ngroups, const SplPlan& sp,
-                          bool count, uint32_t* next_group, uint32_t* scratch, uint32_t node_count) {
+                          bool count, uint32_t* next_group, uint32_t* scratch, uint32_t wide_count) {
     if (sp.use)
         HR_HIP(ctx, launch_shadows_spl(d_jobs, n, ngroups, sp.lg2s, sp.adaptive, ctx->d_counters, ctx->stream,
-                                       next_group, scratch, ctx->resident_blocks, node_count));
+                                       next_group, scratch, ctx->resident_blocks, wide_count));
     else
         HR_HIP(ctx, launch_shadows(d_jobs, n, ngroups, ctx->d_counters, count, ctx->stream, next_group,
                                    ctx->resident_blocks));
@@ -702,7 +706,6 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     check(scene_alloc(s, &b.lvl, nn + 2 * (size_t)REFIT_LEVELS + 1));
     check(scene_alloc(s, &b.kbox, nn * 2));
     check(scene_alloc(s, &b.nodes, nn));
-    check(scene_alloc(s, &b.nodes4, nn));
     check(scene_alloc(s, &b.stris, n));
     check(scene_alloc(s, &b.tri_prim, n));
     check(scene_alloc(s, &b.depth, 1));
@@ -757,6 +760,16 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
 
     he = hipEventRecord(ctx->ev[0], ctx->stream);
     if (he == hipSuccess) he = lbvh_build(b, ctx->stream);
+    if (he == hipSuccess && n > 0) {  // the 4-wide nodes, dense: as many as the build chose
+        st = HR_OK;
+        check(scene_alloc(s, &b.nodes4, b.wide_count));
+        check(scene_alloc(s, &b.wsrc, b.wide_count));
+        if (st != HR_OK) {
+            hr_scene_destroy(s);
+            return st;
+        }
+        he = lbvh_build_wide(b, ctx->stream);
+    }
     if (he == hipSuccess) he = hipEventRecord(ctx->ev[1], ctx->stream);
     if (he == hipSuccess) he = hipEventSynchronize(ctx->ev[1]);
     if (he == hipSuccess) he = hipEventElapsedTime(&s->ms_build, ctx->ev[0], ctx->ev[1]);
@@ -908,6 +921,17 @@ hr_status hr_scene_info(const hr_scene* s, uint32_t* tri_count, uint32_t* node_c
     return HR_OK;
 }
 
+hr_status hr_scene_memory(const hr_scene* s, uint32_t* wide_count, uint64_t* device_bytes) {
+    if (!s) return HR_E_INVALID;
+    if (wide_count) *wide_count = s->bb.wide_count;
+    if (device_bytes) {
+        uint64_t t = 0;
+        for (size_t b : s->alloc_bytes) t += b;
+        *device_bytes = t;
+    }
+    return HR_OK;
+}
+
 hr_status hr_scene_export(const hr_scene* s, uint32_t which, void* dst, size_t bytes) {
     if (!s) return HR_E_INVALID;
     hr_context* ctx = s->ctx;
@@ -922,7 +946,7 @@ hr_status hr_scene_export(const hr_scene* s, uint32_t which, void* dst, size_t b
         case 1: src = s->bb.stris; n = (size_t)nt * sizeof(STri); break;
         case 2: src = s->bb.keys; n = (size_t)nt * 4; break;
         case 3: src = s->bb.ids; n = (size_t)nt * 4; break;
-        case 5: src = s->bb.nodes4; n = (size_t)s->node_count * sizeof(TNode4); break;
+        case 5: src = s->bb.nodes4; n = (size_t)s->bb.wide_count * sizeof(TNode4); break;
         case 4: {
             if (nt >= 2) {
                 size_t m = nt - 1;
@@ -1008,7 +1032,7 @@ hr_status hr_trace_shadows(hr_scene* s, const hr_gbuffer* gb, const hr_light* li
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, 1, ngroups, sp, count, ctx->d_next_group,
-                           reinterpret_cast<uint32_t*>(ctx->work), s->node_count));
+                           reinterpret_cast<uint32_t*>(ctx->work), s->bb.wide_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1045,7 +1069,7 @@ hr_status hr_trace_shadows_noalloc(hr_scene* s, const hr_gbuffer* gb, const hr_l
     uint32_t* d_bits = reinterpret_cast<uint32_t*>(base + NOALLOC_BITS_OFFSET);
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, launch_store_job(job, d_job, ctx->stream));
-    HR_TRY(enqueue_shadows(ctx, d_job, 1, ngroups, sp, fp.count, d_bands, d_bits, s->node_count));
+    HR_TRY(enqueue_shadows(ctx, d_job, 1, ngroups, sp, fp.count, d_bands, d_bits, s->bb.wide_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1166,7 +1190,7 @@ hr_status hr_trace_shadows_batch(hr_context* ctx, const hr_frame_desc* frames, u
     HR_TRY(frame_begin(ctx, stats != nullptr));
     HR_HIP(ctx, hipMemcpyAsync(ctx->d_jobs, jobs.data(), n * sizeof(ShadowJob), hipMemcpyHostToDevice, ctx->stream));
     HR_TRY(enqueue_shadows(ctx, ctx->d_jobs, n, ngroups, sp, count, ctx->d_next_group,
-                           reinterpret_cast<uint32_t*>(ctx->work), plans[0].scene->node_count));
+                           reinterpret_cast<uint32_t*>(ctx->work), plans[0].scene->bb.wide_count));
     return frame_end(ctx, stats, stats ? &stats->ms_shadow : nullptr);
 }
 
@@ -1249,7 +1273,7 @@ hr_status hr_graph_capture(hr_context* ctx, const hr_frame_desc* frames, uint32_
     if (le != hipSuccess) st = hip_fail(ctx, le, "launch");
     if (st == HR_OK)
         st = enqueue_shadows(ctx, g->d_jobs, n, ngroups, sp, count, g->d_next, static_cast<uint32_t*>(g->work),
-                             plans[0].scene->node_count);
+                             plans[0].scene->bb.wide_count);
     if (st == HR_OK) st = enqueue_shading(ctx, plans, static_cast<char*>(g->work), mb);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);

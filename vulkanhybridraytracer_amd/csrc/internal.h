@@ -91,7 +91,8 @@ struct DevScene {
     const float4* irr;
     const uint32_t* texels;     // material textures, RGBA8
     const TexInfo* tex;
-    uint32_t node_count, tri_count, nmat, prim_count;
+    uint32_t node_count, tri_count, nmat, prim_count;  // node_count: binary internal nodes (1 for one triangle)
+    uint32_t wide_count;                               // 4-wide nodes
     uint32_t sky_w, sky_h, irr_w, irr_h;
     uint32_t ntex;
 };
@@ -166,7 +167,7 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive);
 // 32-bit node addressing), ignored for several frames
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                               bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
-                              uint32_t* scratch, uint32_t resident_blocks, uint32_t node_count);
+                              uint32_t* scratch, uint32_t resident_blocks, uint32_t wide_count);
 // jobs_dev: njobs ShadowJob in device memory; next_group: BAND_COUNTER_BYTES
 // of device memory holding the SHADOW_QUEUES band counters, BAND_STRIDE words
 // apart (reset by the launch); resident_blocks: the persistent grid size
@@ -254,14 +255,21 @@ struct BuildBuffers {
     uint32_t refit_levels;  // host side: depth buckets in use, set by the build
     float4* kbox;       // internal node boxes x 2
     TNode* nodes;
-    TNode4* nodes4;     // node_count slots, filled at the wide roots
+    TNode4* nodes4;     // wide_count 4-wide nodes, dense (the root first), allocated between
+                        // lbvh_build and lbvh_build_wide
+    uint4* wsrc;        // per 4-wide node, the binary source of each child slot: (node << 1) | side, or EMPTY4
+    uint32_t wide_count;  // host side: set by lbvh_build
     STri* stris;
     uint32_t* tri_prim;
     uint32_t* depth;    // 1 uint
 };
 size_t radix_hist_words(uint32_t n);
-hipError_t lbvh_build(BuildBuffers& b, hipStream_t st);   // full build
-hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st);   // new models, same topology
+// full build in two calls: lbvh_build makes the binary tree and chooses the
+// 4-wide nodes (sets wide_count, synchronising the stream); the caller then
+// allocates nodes4 and wsrc with wide_count entries, and lbvh_build_wide fills them
+hipError_t lbvh_build(BuildBuffers& b, hipStream_t st);
+hipError_t lbvh_build_wide(BuildBuffers& b, hipStream_t st);
+hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st);   // new models, same topology (binary and 4-wide)
 
 // ---- denoiser kernels (denoise.hip) ------------------------------------------------
 // Activations are BF16 NHWC images of N x H x W pixels; a tensor is a channel

@@ -442,21 +442,27 @@ __global__ void k_collapse_consumed(uint32_t* count) {
     if (threadIdx.x == 0) *count = 0u;
 }
 
+// The choice is recorded, not yet written as a node: wide roots are flagged
+// in `flag` (by binary index) and each child slot's binary source (node << 1
+// | side) goes to src[i]; lbvh_build_wide then numbers the wide roots densely
+// in binary-index order and writes the nodes.
 __global__ __launch_bounds__(256) void k_collapse_level(BuildBuffers b, const uint32_t* roots,
                                                         const uint32_t* nroots, uint32_t* next,
-                                                        uint32_t* nnext) {
+                                                        uint32_t* nnext, uint32_t* flag, uint4* src_out) {
     const uint32_t n_in = *nroots;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_in; t += gridDim.x * blockDim.x) {
         const uint32_t i = roots[t];
-        uint32_t code[4];
+        uint32_t code[4], src[4];
         float4 lo[4], hi[4];
         const TNode nd = b.nodes[i];
         code[0] = __float_as_uint(nd.a.w);
         lo[0] = nd.a;
         hi[0] = nd.b;
+        src[0] = i << 1;
         code[1] = __float_as_uint(nd.b.w);
         lo[1] = nd.c;
         hi[1] = nd.d;
+        src[1] = (i << 1) | 1u;
         uint32_t m = 2;
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -473,45 +479,145 @@ __global__ __launch_bounds__(256) void k_collapse_level(BuildBuffers b, const ui
                 }
             }
             if (best < 0) break;
-            const TNode g = b.nodes[code[best]];
+            const uint32_t opened = code[best];
+            const TNode g = b.nodes[opened];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (k == best) {  // the first grandchild takes its place, the second is appended
                     code[k] = __float_as_uint(g.a.w);
                     lo[k] = g.a;
                     hi[k] = g.b;
+                    src[k] = opened << 1;
                 }
                 if ((uint32_t)k == m) {
                     code[k] = __float_as_uint(g.b.w);
                     lo[k] = g.c;
                     hi[k] = g.d;
+                    src[k] = (opened << 1) | 1u;
                 }
             }
             ++m;
         }
-        const float inf = __uint_as_float(0x7f800000u);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if ((uint32_t)k >= m) {
-                code[k] = EMPTY4;
-                lo[k] = make_float4(inf, inf, inf, 0.0f);
-                hi[k] = make_float4(-inf, -inf, -inf, 0.0f);
-            }
-        }
-        TNode4 q;
-        q.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-        q.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-        q.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-        q.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-        q.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-        q.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-        q.code = make_uint4(code[0], code[1], code[2], code[3]);
-        q.pad = make_uint4(0u, 0u, 0u, 0u);
-        b.nodes4[i] = q;
+        for (int k = 0; k < 4; ++k)
+            if ((uint32_t)k >= m) src[k] = EMPTY4;
+        flag[i] = 1u;
+        src_out[i] = make_uint4(src[0], src[1], src[2], src[3]);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if ((uint32_t)k < m && !(code[k] & LEAF_BIT)) next[atomicAdd(nnext, 1u)] = code[k];
     }
+}
+
+// A 4-wide node from the binary sources of its child slots: child k is side
+// (src & 1) of binary node src >> 1 (its box and code as that node stores
+// them; internal codes mapped to dense indices by `dense`), EMPTY4 slots get
+// the inverted box. `binary` (the binary node the wide node is rooted at) is
+// kept in the padding word.
+__device__ TNode4 wide_node(const TNode* __restrict__ nodes, uint4 src, const uint32_t* __restrict__ dense,
+                            uint32_t binary) {
+    const uint32_t s[4] = {src.x, src.y, src.z, src.w};
+    const float inf = __uint_as_float(0x7f800000u);
+    float4 lo[4], hi[4];
+    uint32_t code[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (s[k] == EMPTY4) {
+            code[k] = EMPTY4;
+            lo[k] = make_float4(inf, inf, inf, 0.0f);
+            hi[k] = make_float4(-inf, -inf, -inf, 0.0f);
+            continue;
+        }
+        const TNode nd = nodes[s[k] >> 1];
+        const bool second = (s[k] & 1u) != 0u;
+        lo[k] = second ? nd.c : nd.a;
+        hi[k] = second ? nd.d : nd.b;
+        code[k] = __float_as_uint(second ? nd.b.w : nd.a.w);
+        if (dense && !(code[k] & LEAF_BIT)) code[k] = dense[code[k]];
+    }
+    TNode4 q;
+    q.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+    q.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+    q.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+    q.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+    q.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+    q.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+    q.code = make_uint4(code[0], code[1], code[2], code[3]);
+    q.pad = make_uint4(binary, 0u, 0u, 0u);
+    return q;
+}
+
+// Dense numbering of the wide roots: an exclusive scan of the flags over
+// tiles of SCAN_THREADS * 4 words (tile sums, their scan, then each tile's
+// own scan from its offset); the total goes to *total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_sums(const uint32_t* a, uint32_t len, uint32_t* sums) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const uint32_t base = blockIdx.x * SCAN_THREADS * 4u + threadIdx.x * 4u;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += base + k < len ? a[base + k] : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += (uint32_t)__shfl_down((int)s, off, 64);
+    if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < SCAN_THREADS / 64; ++w) t += wsum[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(uint32_t* a, uint32_t len, const uint32_t* offs,
+                                                            uint32_t ntiles, uint32_t* total) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t i0 = blockIdx.x * SCAN_THREADS * 4u + tid * 4u;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < len ? a[i0 + k] : 0u;
+    const uint32_t s = v[0] + v[1] + v[2] + v[3];
+    uint32_t inc = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    uint32_t run = offs[blockIdx.x] + before + inc - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (i0 + k < len) a[i0 + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == ntiles - 1u && tid == SCAN_THREADS - 1) *total = run;
+}
+
+// the 4-wide nodes at their dense slots (dense = the scanned flags: binary
+// node i is a wide root when the prefix grows after it)
+__global__ __launch_bounds__(256) void k_wide_fill(BuildBuffers b, uint32_t nn, const uint32_t* dense,
+                                                   const uint4* src, const uint32_t* total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const uint32_t d = dense[i];
+    const uint32_t d_next = i + 1u < nn ? dense[i + 1u] : *total;
+    if (d_next == d) return;
+    const uint4 s = src[i];
+    b.nodes4[d] = wide_node(b.nodes, s, dense, i);
+    b.wsrc[d] = s;
+}
+
+// refit: the same 4-wide nodes with boxes from the refitted binary nodes
+// (codes unchanged)
+__global__ __launch_bounds__(256) void k_wide_refit(BuildBuffers b) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= b.wide_count) return;
+    const TNode4 old = b.nodes4[d];
+    TNode4 q = wide_node(b.nodes, b.wsrc[d], nullptr, old.pad.x);
+    q.code = old.code;
+    b.nodes4[d] = q;
 }
 
 __global__ __launch_bounds__(256) void k_gather(BuildBuffers b) {
@@ -582,7 +688,8 @@ hipError_t refit_order(BuildBuffers& b, hipStream_t st) {
     return e;
 }
 
-hipError_t finish(BuildBuffers& b, hipStream_t st) {
+// binary boxes bottom-up, the binary nodes, the sorted triangles and the depth
+hipError_t finish_binary(BuildBuffers& b, hipStream_t st) {
     uint32_t n = b.tri_count;
     if (n >= 2) {
         const uint32_t nn = n - 1u;
@@ -591,27 +698,50 @@ hipError_t finish(BuildBuffers& b, hipStream_t st) {
     }
     uint32_t nn = n >= 2 ? n - 1 : 1;
     k_layout<<<blocks(nn, 256), 256, 0, st>>>(b);
-    // wide levels: roots alternate between the radix sort's spare key and id
-    // arrays (n entries each; a level has fewer than n roots), counts in the
-    // histogram buffer. A wide level descends at least one binary level, so
-    // as many levels as the binary tree has internal-node depths reach every
-    // leaf (one for a single triangle).
-    {
-        uint32_t* roots[2] = {b.keys_tmp, b.ids_tmp};
-        uint32_t* counts = b.hist;
-        const uint32_t grid = std::min(blocks(nn, 256), 1024u);
-        const uint32_t levels = std::max(b.refit_levels, 1u);
-        k_collapse_begin<<<1, 64, 0, st>>>(roots[0], counts);
-        for (uint32_t level = 0; level < levels; ++level) {
-            const uint32_t a = level & 1u;
-            k_collapse_level<<<grid, 256, 0, st>>>(b, roots[a], counts + a, roots[a ^ 1u], counts + (a ^ 1u));
-            k_collapse_consumed<<<1, 64, 0, st>>>(counts + a);
-        }
-    }
     k_gather<<<blocks(n, 256), 256, 0, st>>>(b);
     k_fill_u32<<<1, 64, 0, st>>>(b.depth, 1, 0u);
     k_depth<<<blocks(n, 256), 256, 0, st>>>(b);
     return hipGetLastError();
+}
+
+// Scratch of the wide-node choice, free once the binary nodes are laid out:
+// the depth flags (node_count words) hold the wide-root flags and then their
+// dense numbers, the internal-node boxes (32 bytes per node) the child
+// sources (16 bytes per node), the radix histogram the tile sums and total.
+uint32_t* wide_dense(BuildBuffers& b) { return b.flags; }
+uint4* wide_src(BuildBuffers& b) { return reinterpret_cast<uint4*>(b.kbox); }
+uint32_t* wide_total(BuildBuffers& b) { return b.hist + 4; }
+uint32_t* wide_sums(BuildBuffers& b) { return b.hist + 64; }
+
+// The wide roots and their child sources, level by level from the root:
+// roots alternate between the radix sort's spare key and id arrays (n entries
+// each; a level has fewer than n roots), counts in the histogram buffer. A
+// wide level descends at least one binary level, so as many levels as the
+// binary tree has internal-node depths reach every leaf (one for a single
+// triangle). Then the dense numbering; its total is read back.
+hipError_t choose_wide(BuildBuffers& b, hipStream_t st) {
+    const uint32_t n = b.tri_count;
+    const uint32_t nn = n >= 2 ? n - 1 : 1;
+    uint32_t* roots[2] = {b.keys_tmp, b.ids_tmp};
+    uint32_t* counts = b.hist;
+    uint32_t* dense = wide_dense(b);
+    k_fill_u32<<<blocks(nn, 256), 256, 0, st>>>(dense, nn, 0u);
+    const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+    const uint32_t levels = std::max(b.refit_levels, 1u);
+    k_collapse_begin<<<1, 64, 0, st>>>(roots[0], counts);
+    for (uint32_t level = 0; level < levels; ++level) {
+        const uint32_t a = level & 1u;
+        k_collapse_level<<<grid, 256, 0, st>>>(b, roots[a], counts + a, roots[a ^ 1u], counts + (a ^ 1u), dense,
+                                                wide_src(b));
+        k_collapse_consumed<<<1, 64, 0, st>>>(counts + a);
+    }
+    const uint32_t ntiles = blocks(nn, SCAN_THREADS * 4);
+    k_tile_sums<<<ntiles, SCAN_THREADS, 0, st>>>(dense, nn, wide_sums(b));
+    k_rs_scan<<<1, SCAN_THREADS, 0, st>>>(wide_sums(b), ntiles);
+    k_tile_scan<<<ntiles, SCAN_THREADS, 0, st>>>(dense, nn, wide_sums(b), ntiles, wide_total(b));
+    hipError_t e = hipMemcpyAsync(&b.wide_count, wide_total(b), sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
 }
 
 }  // namespace
@@ -620,6 +750,7 @@ size_t radix_hist_words(uint32_t n) { return 256u * (size_t)blocks(n > 0 ? n : 1
 
 hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
     uint32_t n = b.tri_count;
+    b.wide_count = 0;
     if (n == 0) return hipSuccess;
     k_init_bounds<<<1, 64, 0, st>>>(b.bounds);
     k_transform<<<blocks(n, 256), 256, 0, st>>>(b, 1);
@@ -631,14 +762,27 @@ hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
         e = refit_order(b, st);
         if (e != hipSuccess) return e;
     }
-    return finish(b, st);
+    e = finish_binary(b, st);
+    if (e != hipSuccess) return e;
+    return choose_wide(b, st);
+}
+
+hipError_t lbvh_build_wide(BuildBuffers& b, hipStream_t st) {
+    const uint32_t n = b.tri_count;
+    if (n == 0) return hipSuccess;
+    const uint32_t nn = n >= 2 ? n - 1 : 1;
+    k_wide_fill<<<blocks(nn, 256), 256, 0, st>>>(b, nn, wide_dense(b), wide_src(b), wide_total(b));
+    return hipGetLastError();
 }
 
 hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st) {
     uint32_t n = b.tri_count;
     if (n == 0) return hipSuccess;
     k_transform<<<blocks(n, 256), 256, 0, st>>>(b, 0);
-    return finish(b, st);
+    hipError_t e = finish_binary(b, st);
+    if (e != hipSuccess) return e;
+    if (b.wide_count) k_wide_refit<<<blocks(b.wide_count, 256), 256, 0, st>>>(b);
+    return hipGetLastError();
 }
 
 }  // namespace hr

@@ -527,7 +527,7 @@ hipError_t launch_rt_wavefront(const DevScene& s, const FrameArgs& f, float4* ac
             k_rt_reset<<<1, 64, 0, st>>>(w, depth, next_item);
             k_rt_closest<<<nbT, BLOCK, 0, st>>>(s, f, w, depth, counters);
             if (nbW) k_rt_clear_bits<<<nbW, 256, 0, st>>>(w, f.nl, lg2s);
-            if ((uint64_t)s.node_count * sizeof(TNode4) <= 0xFFFFFFFFull)
+            if ((uint64_t)s.wide_count * sizeof(TNode4) <= 0xFFFFFFFFull)
                 k_rt_shadows<true><<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
             else
                 k_rt_shadows<false><<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);

@@ -756,9 +756,9 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
 
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                               bool adaptive, unsigned long long* counters, hipStream_t st, uint32_t* next_item,
-                              uint32_t* scratch, uint32_t resident_blocks, uint32_t node_count) {
+                              uint32_t* scratch, uint32_t resident_blocks, uint32_t wide_count) {
     const bool multi = njobs > 1u;
-    const bool off32 = !multi && (uint64_t)node_count * sizeof(TNode4) <= 0xFFFFFFFFull;
+    const bool off32 = !multi && (uint64_t)wide_count * sizeof(TNode4) <= 0xFFFFFFFFull;
     const uint64_t items = (uint64_t)ngroups << lg2s;
     if (items >= (1ull << 32)) return hipErrorInvalidValue;
     const uint32_t words = (uint32_t)((items + 31u) >> 5);

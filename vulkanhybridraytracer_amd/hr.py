@@ -90,6 +90,7 @@ def load(path=LIB_PATH):
         "hr_asset_destroy": (None, [vp]),
         "hr_scene_destroy": (None, [vp]),
         "hr_scene_info": (st, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(C.c_float)]),
+        "hr_scene_memory": (st, [vp, C.POINTER(u32), C.POINTER(u64)]),
         "hr_scene_export": (st, [vp, u32, vp, sz]),
         "hr_gbuffer_primary": (st, [vp, C.POINTER(abi.hr_camera), C.POINTER(abi.hr_gbuffer),
                                     C.POINTER(abi.hr_stats)]),
@@ -266,12 +267,18 @@ class HybridScene:
         self.ctx.check(self.ctx.L.hr_scene_info(self.h, C.byref(t), C.byref(n), C.byref(d), C.byref(ms)))
         return dict(tri_count=t.value, node_count=n.value, depth=d.value, ms_build=ms.value)
 
+    def memory(self):
+        """4-wide node count and the scene's device bytes"""
+        w, b = C.c_uint32(), C.c_uint64()
+        self.ctx.check(self.ctx.L.hr_scene_memory(self.h, C.byref(w), C.byref(b)))
+        return dict(wide_count=w.value, device_bytes=b.value)
+
     def export(self, which):
         info = self.info()
         n = info["tri_count"]
         size, dt = {0: (info["node_count"], abi.NODE_DTYPE), 1: (n, abi.STRI_DTYPE), 2: (n, np.uint32),
                     3: (n, np.uint32), 4: (max(n - 1, 0) * 4, np.uint32),
-                    5: (info["node_count"] * 32, np.uint32)}[which]
+                    5: (self.memory()["wide_count"] * 32, np.uint32)}[which]
         out = np.zeros(size, dtype=dt)
         self.ctx.check(self.ctx.L.hr_scene_export(self.h, which, out.ctypes.data, out.nbytes))
         return out
