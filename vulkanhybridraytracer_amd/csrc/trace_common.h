@@ -65,9 +65,15 @@ constexpr int REFILL_MIN_ITEMS = 40;
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const float4 __attribute__((address_space(1)))* gf4;
 typedef const uint4 __attribute__((address_space(1)))* gu4;
+// read-only for the kernel's lifetime: a wave-uniform address loads through
+// the scalar cache
+typedef const float4 __attribute__((address_space(4)))* cf4;
+typedef const uint4 __attribute__((address_space(4)))* cu4;
 #else
 typedef const float4* gf4;
 typedef const uint4* gu4;
+typedef const float4* cf4;
+typedef const uint4* cu4;
 #endif
 __device__ __forceinline__ gf4 gptr(const void* p) { return (gf4)(p); }
 
@@ -149,7 +155,21 @@ __device__ __forceinline__ uint32_t wide_step_any(const TNode4* nodes4, uint32_t
                                                   uint32_t* spill, int& sp, Cnt& c) {
     float4 nx, fx, ny, fy, nz, fz;
     uint4 cd;
-    if (OFF32) {
+    const uint32_t u = __builtin_amdgcn_readfirstlane(cur);
+    if (OFF32 && __ballot(cur != u) == 0ull) {
+        // every walking lane is at node u: it is read once, through the
+        // scalar cache, and each lane picks its near and far planes
+        const cf4 q = (cf4)(nodes4 + u);
+        const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+        cd = ((cu4)q)[6];
+        const bool sx = pl.x != 0u, sy = pl.y != 32u, sz = pl.z != 64u;
+        nx = sx ? hx : lx;
+        fx = sx ? lx : hx;
+        ny = sy ? hy : ly;
+        fy = sy ? ly : hy;
+        nz = sz ? hz : lz;
+        fz = sz ? lz : hz;
+    } else if (OFF32) {
         const char* b = reinterpret_cast<const char*>(nodes4);
         const uint32_t o = cur << 7;
         nx = *gptr(b + (o + pl.x));
