@@ -54,9 +54,11 @@ constexpr uint32_t DONE = 0xFFFFFFFFu;  // no node: no leaf code reaches it (few
 // walking, so lanes whose ray ended or that hold two leaves get back to work
 // (leaf tests, new rays) instead of idling; results do not depend on it.
 // consume_rays (queues of closest-hit and hit-shadow rays) and item_loop
-// (sample-parallel shadow items) have their own.
+// (sample-parallel shadow items) have their own. The item loop refills early:
+// lanes that start their rays together walk the same nodes, which a
+// wave-uniform node step reads through the scalar cache.
 constexpr int REFILL_MIN_LANES = 16;
-constexpr int REFILL_MIN_ITEMS = 40;
+constexpr int REFILL_MIN_ITEMS = 56;
 
 // Node and triangle arrays live in global memory; reading them through
 // global-address-space pointers gives global (not flat) loads, whose waits do
