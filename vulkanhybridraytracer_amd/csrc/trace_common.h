@@ -959,19 +959,33 @@ __device__ __forceinline__ void item_loop(Src& src, uint32_t nitems, uint32_t* n
         if (lf != DONE) {
             const uint32_t first = lf & 0x07FFFFFFu;
             const uint32_t end = first + ((lf >> 27) & 0xFu) + 1u;
-            // Two triangles per step, both loaded before either test: a leaf
-            // costs half as many memory round trips (the second of an odd
-            // leaf's last pair repeats the first).
-            for (uint32_t k = first; k < end; k += 2u) {
-                const gf4 tp = gptr(tris + k), tq = gptr(tris + (k + 1u < end ? k + 1u : k));
-                const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-                const float4 w0 = tq[0], w1 = tq[1], w2 = tq[2];
-                float t, b1, b2;
-                const bool h0 = intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2);
-                const bool h1 = intersect_tri(r, ld3(w0), ld3(w1), ld3(w2), tmin, tmax, t, b1, b2);
-                if (h0 || h1) {
-                    hit = true;
-                    break;
+            const uint32_t ul = __builtin_amdgcn_readfirstlane(lf);
+            if (OFF32 && __ballot(lf != ul) == 0ull) {
+                // every lane at one leaf: its triangles through the scalar cache
+                const uint32_t ufirst = ul & 0x07FFFFFFu;
+                const uint32_t uend = ufirst + ((ul >> 27) & 0xFu) + 1u;
+                for (uint32_t k = ufirst; k < uend; ++k) {
+                    const cf4 tp = (cf4)(tris + k);
+                    const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                    float t, b1, b2;
+                    if (intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2)) hit = true;
+                    if (__ballot(!hit) == 0ull) break;
+                }
+            } else {
+                // Two triangles per step, both loaded before either test: a leaf
+                // costs half as many memory round trips (the second of an odd
+                // leaf's last pair repeats the first).
+                for (uint32_t k = first; k < end; k += 2u) {
+                    const gf4 tp = gptr(tris + k), tq = gptr(tris + (k + 1u < end ? k + 1u : k));
+                    const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                    const float4 w0 = tq[0], w1 = tq[1], w2 = tq[2];
+                    float t, b1, b2;
+                    const bool h0 = intersect_tri(r, ld3(v0), ld3(v1), ld3(v2), tmin, tmax, t, b1, b2);
+                    const bool h1 = intersect_tri(r, ld3(w0), ld3(w1), ld3(w2), tmin, tmax, t, b1, b2);
+                    if (h0 || h1) {
+                        hit = true;
+                        break;
+                    }
                 }
             }
             if (hit) {
