@@ -9,10 +9,17 @@
 
 #include "internal.h"
 
+// frames of a batch are shaded on this many streams at once (forked from and
+// joined back into the context's stream), each with its own workspace
+constexpr int SHADE_STREAMS = 4;
+
 struct hr_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t side[SHADE_STREAMS] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[SHADE_STREAMS] = {};
     unsigned long long* d_counters = nullptr;
     uint32_t* d_next_group = nullptr;  // work counter of the persistent shadow kernel
     uint32_t cu_count = 256;

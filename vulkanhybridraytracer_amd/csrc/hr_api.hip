@@ -4,6 +4,7 @@
 // aborted on), the upload of the scene inputs in the layouts the kernels read,
 // the per-primitive constants (normal matrices), and kernel launches on the
 // context's stream. No computation of the hybrid pass happens on the host.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -500,6 +501,11 @@ hr_status hr_context_create(int device, hr_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev[1]);
+    for (int k = 0; k < SHADE_STREAMS && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&ctx->d_counters, C_COUNT * C_STRIDE * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&ctx->d_next_group, BAND_COUNTER_BYTES);
     if (e == hipSuccess) {
@@ -529,6 +535,14 @@ void hr_context_destroy(hr_context* ctx) {
     if (ctx->work) (void)hipFree(ctx->work);
     for (hipEvent_t e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < SHADE_STREAMS; ++k) {
+        if (ctx->side[k]) {
+            (void)hipStreamSynchronize(ctx->side[k]);
+            (void)hipStreamDestroy(ctx->side[k]);
+        }
+        if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
+    }
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1140,25 +1154,84 @@ hr_status plan_batch(hr_context* ctx, const hr_frame_desc* frames, uint32_t n, b
 
 // Kernels of a batch's shading pass, frames one after another on a shared
 // workspace at base (laid out per frame).
-hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base, uint32_t budget_mb) {
-    for (const FramePlan& fp : plans) {
+// How the frames of a batch share the shading streams: frames by decreasing
+// work (pixels x paths per pixel; the lower index first on ties), each to the
+// stream with the least work so far; each stream's workspace holds its
+// largest frame at budget_mb / streams, the streams' workspaces one after
+// another (256-byte aligned). total = 0: a frame does not fit.
+struct ShadeSplit {
+    uint32_t streams = 1;
+    std::vector<uint32_t> stream_of;
+    size_t offset[SHADE_STREAMS] = {};
+    size_t total = 0;
+};
+ShadeSplit split_shading(hr_context* ctx, const std::vector<FramePlan>& plans, uint32_t budget_mb) {
+    ShadeSplit sp;
+    const uint32_t n = (uint32_t)plans.size();
+    sp.streams = n < (uint32_t)SHADE_STREAMS ? (n > 0 ? n : 1u) : (uint32_t)SHADE_STREAMS;
+    sp.stream_of.assign(n, 0u);
+    std::vector<uint64_t> cost(n);
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        cost[i] = (uint64_t)plans[i].f.width * plans[i].f.height * (plans[i].f.spp ? plans[i].f.spp : 1u);
+        order[i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    uint64_t load[SHADE_STREAMS] = {};
+    size_t need[SHADE_STREAMS] = {};
+    const uint32_t mb = budget_mb / sp.streams > 64u ? budget_mb / sp.streams : 64u;
+    for (uint32_t i : order) {
+        uint32_t s = 0;
+        for (uint32_t k = 1; k < sp.streams; ++k)
+            if (load[k] < load[s]) s = k;
+        sp.stream_of[i] = s;
+        load[s] += cost[i];
         WaveArgs w;
-        wave_layout(ctx, fp.f, w, base, budget_mb);
+        const size_t b = wave_layout(ctx, plans[i].f, w, nullptr, mb);
+        if (b == 0) return ShadeSplit{};
+        need[s] = b > need[s] ? b : need[s];
+    }
+    size_t off = 0;
+    for (uint32_t k = 0; k < sp.streams; ++k) {
+        sp.offset[k] = off;
+        off += (need[k] + 255u) & ~(size_t)255u;
+    }
+    sp.total = off;
+    return sp;
+}
+
+// Shading of a batch's frames: on the context's stream for one frame, else
+// forked over the split's streams and joined back (in a captured graph the
+// branches become parallel paths).
+hr_status enqueue_shading(hr_context* ctx, const std::vector<FramePlan>& plans, char* base, uint32_t budget_mb) {
+    const ShadeSplit sp = split_shading(ctx, plans, budget_mb);
+    if (sp.total == 0) return fail(ctx, HR_E_UNSUPPORTED, "reflection slab too large");
+    const uint32_t mb = budget_mb / sp.streams > 64u ? budget_mb / sp.streams : 64u;
+    const bool fork = sp.streams > 1u;
+    if (fork) {
+        HR_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+        for (uint32_t k = 0; k < sp.streams; ++k) HR_HIP(ctx, hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+    }
+    for (size_t i = 0; i < plans.size(); ++i) {
+        const FramePlan& fp = plans[i];
+        const uint32_t s = sp.stream_of[i];
+        WaveArgs w;
+        wave_layout(ctx, fp.f, w, base + sp.offset[s], mb);
         HR_HIP(ctx, launch_shade(dev_scene(fp.scene), fp.f, fp.g, fp.shadow, reinterpret_cast<float4*>(fp.lit),
-                                 reinterpret_cast<float4*>(fp.refl), ctx->d_counters, fp.count, ctx->stream, w));
+                                 reinterpret_cast<float4*>(fp.refl), ctx->d_counters, fp.count,
+                                 fork ? ctx->side[s] : ctx->stream, w));
+    }
+    if (fork) {
+        for (uint32_t k = 0; k < sp.streams; ++k) {
+            HR_HIP(ctx, hipEventRecord(ctx->join_ev[k], ctx->side[k]));
+            HR_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->join_ev[k], 0));
+        }
     }
     return HR_OK;
 }
 
 size_t batch_work_bytes(hr_context* ctx, const std::vector<FramePlan>& plans, uint32_t budget_mb) {
-    size_t m = 0;
-    for (const FramePlan& fp : plans) {
-        WaveArgs w;
-        size_t b = wave_layout(ctx, fp.f, w, nullptr, budget_mb);
-        if (b == 0) return 0;
-        m = b > m ? b : m;
-    }
-    return m;
+    return split_shading(ctx, plans, budget_mb).total;
 }
 
 }  // namespace
