@@ -11,7 +11,7 @@
 
 // frames of a batch are shaded on this many streams at once (forked from and
 // joined back into the context's stream), each with its own workspace
-constexpr int SHADE_STREAMS = 4;
+constexpr int SHADE_STREAMS = 8;
 
 struct hr_context {
     int device = 0;
