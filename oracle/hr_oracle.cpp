@@ -765,10 +765,13 @@ static void build_bvh(or_scene& s) {
             cmin = vmin(cmin, c);
             cmax = vmax(cmax, c);
         }
+        // cubic cells, 1024 along the longest extent: a flat scene spends its
+        // leading key bits on its long axes
         v3 ext = sub(cmax, cmin);
-        float sx = ext.x > 0.0f ? 1024.0f / ext.x : 0.0f;
-        float sy = ext.y > 0.0f ? 1024.0f / ext.y : 0.0f;
-        float sz = ext.z > 0.0f ? 1024.0f / ext.z : 0.0f;
+        const float em = std::max(ext.x, std::max(ext.y, ext.z));
+        float sx = ext.x > 0.0f ? 1024.0f / em : 0.0f;
+        float sy = ext.y > 0.0f ? 1024.0f / em : 0.0f;
+        float sz = ext.z > 0.0f ? 1024.0f / em : 0.0f;
         std::vector<uint32_t> codes(n);
         for (uint32_t g = 0; g < n; ++g)
             codes[g] = morton3(quant10(cen[g].x, cmin.x, sx), quant10(cen[g].y, cmin.y, sy),

@@ -104,7 +104,8 @@ def _check_lbvh(os_, sc):
     c = (lo + hi) * np.float32(0.5)
     cmin, cmax = c.min(0), c.max(0)
     ext = cmax - cmin
-    scale = np.where(ext > 0, np.float32(1024) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
+    # one cell size for the three axes: 1024 cells along the longest extent
+    scale = np.where(ext > 0, np.float32(1024) / ext.max(), 0).astype(np.float32)
     q = np.clip(np.floor((c - cmin) * scale), 0, 1023).astype(np.uint32)
 
     def spread(v):

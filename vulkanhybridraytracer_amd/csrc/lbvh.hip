@@ -4,7 +4,8 @@
 // 1. k_transform   world-space triangles (model * position, vk_mesh.cpp:476;
 //                  44-byte vertices, uint32 indices, vk_mesh.cpp:436-442),
 //                  triangle boxes and the centroid bounds (ordered-uint atomics)
-// 2. k_morton      30-bit Morton code of each box centre on a 1024^3 grid
+// 2. k_morton      30-bit Morton code of each box centre on a grid of cubic
+//                  cells, 1024 along the longest extent of the centre bounds
 // 3. radix sort    stable LSD sort of (code, triangle id), four 8-bit digits:
 //                  per-tile digit histograms, one exclusive scan, and a scatter
 //                  whose in-tile ranks come from 64-lane ballot matching, so
@@ -122,14 +123,19 @@ __global__ __launch_bounds__(256) void k_transform(BuildBuffers b, int with_boun
 __global__ __launch_bounds__(256) void k_morton(BuildBuffers b) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= b.tri_count) return;
-    float cmin[3], cmax[3], scale[3];
+    // cubic cells, 1024 along the longest extent of the centre bounds: a
+    // flat scene spends its leading key bits on its long axes, so the upper
+    // levels split the long axes instead of cutting thin slabs
+    float cmin[3], cmax[3], ext[3], scale[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         cmin[a] = ord2f(b.bounds[a]);
         cmax[a] = ord2f(b.bounds[3 + a]);
-        float ext = cmax[a] - cmin[a];
-        scale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+        ext[a] = cmax[a] - cmin[a];
     }
+    const float em = fmaxf(ext[0], fmaxf(ext[1], ext[2]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) scale[a] = ext[a] > 0.0f ? 1024.0f / em : 0.0f;
     float4 lo = b.tbox[2 * (size_t)g], hi = b.tbox[2 * (size_t)g + 1];
     float cx = (lo.x + hi.x) * 0.5f, cy = (lo.y + hi.y) * 0.5f, cz = (lo.z + hi.z) * 0.5f;
     b.keys[g] = morton3(quant10(cx, cmin[0], scale[0]), quant10(cy, cmin[1], scale[1]),
