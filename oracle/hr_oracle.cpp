@@ -11,8 +11,9 @@
  *   miss.rmiss:13-18        environment miss
  * plus the acceleration structure that replaces the driver's BLAS/TLAS
  * (renderer.cpp:1643-1916): a Karras (HPG 2012) LBVH over world-space
- * triangles with 30-bit Morton codes, stable key order, bottom-up boxes and
- * a two-child-box node layout, and a stack traversal with the watertight
+ * triangles with 30-bit Morton codes, stable key order, bottom-up boxes, one
+ * pass of treelet restructuring (Karras and Aila, HPG 2013) and a
+ * two-child-box node layout, and a stack traversal with the watertight
  * ray/triangle test of Woop, Benthin and Wald (JCGT 2013).
  *
  * It is the checker of the GPU library and the CPU baseline of the benchmark:
@@ -288,7 +289,8 @@ enum : int {
     PLANT_RECT_STRATA = 5,    /* sampleRect rows divided by n instead of m */
     PLANT_TIR_REFRACT = 6,    /* total internal reflection answered by refract() */
     PLANT_ADAPTIVE_STOP = 7,  /* stopping rule against N^2 instead of N^3 */
-    PLANT_NESTED_ORDER = 8    /* nested cells without the bit reversal */
+    PLANT_NESTED_ORDER = 8,   /* nested cells without the bit reversal */
+    PLANT_TREELET_COST = 9    /* treelet split cost without its second part */
 };
 static int g_plant = PLANT_NONE;
 
@@ -666,6 +668,182 @@ static void refit_boxes(or_scene& s) {
         }
     }
 }
+/* ---- treelet restructuring (Karras and Aila, HPG 2013) ---------------------
+ * One bottom-up pass over the Karras tree: every internal node N, after the
+ * nodes below it, roots a treelet grown from N's two children by replacing,
+ * while it has fewer than TREELET_LEAVES leaves, its internal leaf of largest
+ * half box area (the first in list order on ties) by that node's two
+ * children (the first in its place, the second appended). Over the subsets S
+ * of the treelet leaves (bit j = leaf j), by increasing S:
+ *   cost(S) = SAH_TRI * count(S) * area(S)          if count(S) <= leaf_max
+ *           = SAH_NODE * area(S) + min_P cost(P) + cost(S \ P)   otherwise
+ * with P over the subsets of S that hold S's lowest leaf (submasks in
+ * decreasing order, the first minimum, ties to the split whose halves differ
+ * least in leaf count); a single leaf costs its subtree's cost (a triangle or
+ * a subtree of at most leaf_max triangles: SAH_TRI * count * area). The
+ * treelet is rebuilt in its minimum-cost topology: N keeps the root, the other
+ * internal nodes are the removed ones in removal order, assigned in pre-order
+ * (node, first part, second part). area = dx*dy + dy*dz + dz*dx of the box,
+ * boxes are unions in increasing leaf order. Then leaves are renumbered in
+ * depth-first order (first child first), so every subtree again covers a
+ * contiguous range of the sorted triangles; keys keep the Morton order. */
+constexpr int TREELET_LEAVES = 7;
+constexpr float SAH_NODE = 1.2f, SAH_TRI = 1.0f;
+static int g_treelets = 1;
+
+static inline float half_area(const float* b) {
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+static void restructure_treelet(or_scene& s, uint32_t N, std::vector<float>& cost, std::vector<uint32_t>& cnt) {
+    const uint32_t leaf_max = s.leaf_max;
+    uint32_t tl[TREELET_LEAVES], freed[TREELET_LEAVES - 2];
+    int m = 2, nf = 0;
+    tl[0] = s.kl[N];
+    tl[1] = s.kr[N];
+    while (m < TREELET_LEAVES) {
+        int bi = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < m; ++i) {
+            if (tl[i] & LEAF_BIT) continue;
+            float a = half_area(&s.kbox[(size_t)tl[i] * 6]);
+            if (a > ba) { ba = a; bi = i; }
+        }
+        if (bi < 0) break;
+        uint32_t x = tl[bi];
+        freed[nf++] = x;
+        tl[bi] = s.kl[x];
+        tl[m++] = s.kr[x];
+    }
+    const int ns = 1 << m;
+    float lb[TREELET_LEAVES][6], C[1 << TREELET_LEAVES];
+    uint32_t K[1 << TREELET_LEAVES];
+    int part[1 << TREELET_LEAVES];
+    for (int j = 0; j < m; ++j) {
+        child_box(s, tl[j], lb[j]);
+        uint32_t k = (tl[j] & LEAF_BIT) ? 1u : cnt[tl[j]];
+        K[1 << j] = k;
+        C[1 << j] = ((tl[j] & LEAF_BIT) || k <= leaf_max) ? SAH_TRI * (float)k * half_area(lb[j]) : cost[tl[j]];
+        part[1 << j] = 0;
+    }
+    auto set_box = [&](int S, float* b) {
+        int j = __builtin_ctz((unsigned)S);
+        for (int q = 0; q < 6; ++q) b[q] = lb[j][q];
+        for (++j; j < m; ++j) {
+            if (!((S >> j) & 1)) continue;
+            for (int q = 0; q < 3; ++q) {
+                b[q] = minsel(b[q], lb[j][q]);
+                b[3 + q] = maxsel(b[3 + q], lb[j][3 + q]);
+            }
+        }
+    };
+    for (int S = 1; S < ns; ++S) {
+        if ((S & (S - 1)) == 0) continue;
+        const int low = S & -S, pc = __builtin_popcount((unsigned)S);
+        K[S] = K[low] + K[S ^ low];
+        float best = INFINITY;
+        int bp = 0, bbal = 1 << 30;
+        for (int P = (S - 1) & S; P > 0; P = (P - 1) & S) {
+            if (!(P & low)) continue;
+            const float c = g_plant == PLANT_TREELET_COST ? C[P] : C[P] + C[S ^ P];
+            const int bal = std::abs(2 * __builtin_popcount((unsigned)P) - pc);
+            if (c < best || (c == best && bal < bbal)) { best = c; bp = P; bbal = bal; }
+        }
+        float b[6];
+        set_box(S, b);
+        const float A = half_area(b);
+        C[S] = K[S] <= leaf_max ? SAH_TRI * (float)K[S] * A : SAH_NODE * A + best;
+        part[S] = bp;
+    }
+    /* pre-order list of the internal subsets, then their nodes */
+    int order[TREELET_LEAVES - 1], no = 0, stk[TREELET_LEAVES], sp = 0;
+    stk[sp++] = ns - 1;
+    while (sp) {
+        const int S = stk[--sp];
+        order[no++] = S;
+        const int P = part[S] ? part[S] : (S & -S), Q = S ^ P;
+        if (Q & (Q - 1)) stk[sp++] = Q;
+        if (P & (P - 1)) stk[sp++] = P;
+    }
+    auto code_of = [&](int S) -> uint32_t {
+        if ((S & (S - 1)) == 0) return tl[__builtin_ctz((unsigned)S)];
+        for (int t = 0; t < no; ++t)
+            if (order[t] == S) return t == 0 ? N : freed[t - 1];
+        return 0xFFFFFFFFu;
+    };
+    for (int t = 0; t < no; ++t) {
+        const int S = order[t];
+        const int P = part[S] ? part[S] : (S & -S), Q = S ^ P;
+        const uint32_t id = t == 0 ? N : freed[t - 1];
+        const uint32_t l = code_of(P), r = code_of(Q);
+        s.kl[id] = l;
+        s.kr[id] = r;
+        set_box(S, &s.kbox[(size_t)id * 6]);
+        if (!(l & LEAF_BIT)) s.kparent[l] = id;
+        if (!(r & LEAF_BIT)) s.kparent[r] = id;
+        cost[id] = C[S];
+        cnt[id] = K[S];
+    }
+}
+
+static void restructure(or_scene& s) {
+    const uint32_t n = s.ntri;
+    if (n < 3) return;
+    std::vector<float> cost(n - 1, 0.0f);
+    std::vector<uint32_t> cnt(n - 1, 0u);
+    /* post-order: every node after the nodes below it */
+    std::vector<std::pair<uint32_t, int>> st;
+    st.push_back({0u, 0});
+    while (!st.empty()) {
+        auto& top = st.back();
+        const uint32_t N = top.first;
+        if (top.second == 0) {
+            top.second = 1;
+            if (!(s.kl[N] & LEAF_BIT)) st.push_back({s.kl[N], 0});
+            if (!(s.kr[N] & LEAF_BIT)) st.push_back({s.kr[N], 0});
+            continue;
+        }
+        st.pop_back();
+        restructure_treelet(s, N, cost, cnt);
+    }
+    /* depth-first leaf order: new leaf positions and node ranges */
+    std::vector<uint32_t> ids(n), newpos(n);
+    uint32_t next = 0;
+    std::vector<std::pair<uint32_t, int>> dfs;
+    dfs.push_back({0u, 0});
+    while (!dfs.empty()) {
+        auto& top = dfs.back();
+        const uint32_t c = top.first;
+        if (top.second == 0) {
+            top.second = 1;
+            s.kfirst[c] = next;
+            const uint32_t ch[2] = {s.kl[c], s.kr[c]};
+            /* the second child is visited after the first's whole subtree */
+            dfs.push_back({ch[1], 2});
+            dfs.push_back({ch[0], 2});
+            continue;
+        }
+        if (top.second == 2 && (c & LEAF_BIT)) {
+            dfs.pop_back();
+            newpos[c & ~LEAF_BIT] = next;
+            ids[next++] = s.ids[c & ~LEAF_BIT];
+            continue;
+        }
+        if (top.second == 2) {
+            top.second = 0;
+            continue;
+        }
+        dfs.pop_back();
+        s.klast[c] = next - 1;
+    }
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        if (s.kl[i] & LEAF_BIT) s.kl[i] = LEAF_BIT | newpos[s.kl[i] & ~LEAF_BIT];
+        if (s.kr[i] & LEAF_BIT) s.kr[i] = LEAF_BIT | newpos[s.kr[i] & ~LEAF_BIT];
+    }
+    s.ids.swap(ids);
+}
+
 static void layout_nodes(or_scene& s) {
     uint32_t n = s.ntri;
     s.stris.assign(n, STri{});
@@ -784,6 +962,7 @@ static void build_bvh(or_scene& s) {
     if (n >= 2) {
         karras(s);
         refit_boxes(s);
+        if (g_treelets) restructure(s);
     }
     layout_nodes(s);
     compute_depth(s);
@@ -1333,6 +1512,7 @@ float or_u01_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index, uint3
     return u;
 }
 uint32_t or_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
+void or_set_treelets(int on) { g_treelets = on ? 1 : 0; }
 void or_atan2(const float* y, const float* x, uint32_t n, float* out) {
     for (uint32_t i = 0; i < n; ++i) out[i] = hr_atan2(y[i], x[i]);
 }

@@ -35,6 +35,9 @@ float or_rnd(uint32_t* state);
 void or_pcg32_seq(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out);
 float or_u01_stream(uint32_t pixel_seed, uint32_t purpose, uint32_t index, uint32_t k);
 uint32_t or_morton3(uint32_t x, uint32_t y, uint32_t z);
+/* treelet restructuring in scenes created afterwards (1, the default: as the
+ * GPU library builds; 0: the plain Karras tree, for checks of that stage) */
+void or_set_treelets(int on);
 
 /* math hooks: out must hold n values */
 void or_atan2(const float* y, const float* x, uint32_t n, float* out);

@@ -79,6 +79,7 @@ def lib():
         L.or_u01_stream.argtypes = [u32, u32, u32, u32]
         L.or_morton3.restype = u32
         L.or_morton3.argtypes = [u32, u32, u32]
+        L.or_set_treelets.argtypes = [i32]
         L.or_atan2.argtypes = [vp, vp, u32, vp]
         L.or_asin.argtypes = [vp, u32, vp]
         L.or_sincos2pi.argtypes = [vp, u32, vp, vp]
@@ -406,11 +407,28 @@ def glass_hit(N, rd, ior):
     return out
 
 
+class treelets:
+    """Context manager (tests only): scenes created inside the block build
+    with (on=True, the default of the oracle and of the GPU library) or
+    without the treelet restructuring of the Karras tree."""
+
+    def __init__(self, on):
+        self.on = on
+
+    def __enter__(self):
+        lib().or_set_treelets(1 if self.on else 0)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_treelets(1)
+        return False
+
+
 class planted:
     """Context manager (tests only): the oracle with one deliberate mistake
     (PLANT_* in hr_oracle.cpp) for the duration of the block."""
     GLASS_ETA, GLASS_NORMAL, FRESNEL_NDOTH, GGX_ALPHA, RECT_STRATA, TIR_REFRACT = 1, 2, 3, 4, 5, 6
-    ADAPTIVE_STOP, NESTED_ORDER = 7, 8
+    ADAPTIVE_STOP, NESTED_ORDER, TREELET_COST = 7, 8, 9
 
     def __init__(self, which):
         self.which = which

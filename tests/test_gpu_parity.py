@@ -238,8 +238,8 @@ def _binary_children(nodes, i):
 @pytest.mark.parametrize("leaf_max", [1, 4])
 def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
     """the 4-wide nodes of the timed kernels: stored densely (the root first,
-    then in the order of the binary nodes they are rooted at, about a third
-    as many as binary nodes); from the root they reach exactly the binary
+    then in the order of the binary nodes they are rooted at, a third to a
+    half as many as binary nodes); from the root they reach exactly the binary
     tree's leaves, each child box equal to the binary box of the same subtree,
     empty slots with inverted boxes, and every wide node is the greedy
     largest-area collapse of its binary node. A refit keeps the wide topology
@@ -265,7 +265,7 @@ def check_wide_nodes(hs, LEAF, check_collapse):
     bn = hs.export(0).view(np.float32).reshape(-1, 16)
     w4 = hs.export(5).reshape(-1, 32)
     mem = hs.memory()
-    assert mem["wide_count"] == len(w4) and 0 < len(w4) < 0.5 * len(bn)
+    assert mem["wide_count"] == len(w4) and 0 < len(w4) < 0.6 * len(bn)
     binary_of = w4[:, 28].astype(np.int64)
     assert binary_of[0] == 0 and np.all(np.diff(binary_of) > 0)
     dense_of = {int(b): d for d, b in enumerate(binary_of)}

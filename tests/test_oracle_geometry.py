@@ -2,7 +2,11 @@
   - watertight ray/triangle (Woop, Benthin, Wald 2013): closed forms and
     no leaks through shared edges and vertices of a mesh;
   - LBVH (Karras 2012): Morton codes and stable order recomputed with numpy,
-    structural invariants of the hierarchy and the 64-byte node layout;
+    structural invariants of the hierarchy and the 64-byte node layout, with
+    and without the treelet restructuring (Karras and Aila 2013);
+  - treelet restructuring: the SAH cost never rises, and on sets of up to 7
+    triangles (one treelet) the tree is the cheapest of all binary trees
+    (exhaustive search);
   - traversal (closest hit with (t, id) tie-break, any hit) against brute
     force over every triangle."""
 import numpy as np
@@ -81,7 +85,10 @@ def test_watertight_shared_edges_and_vertices(oracle_lib):
             assert np.all(hits["tri"] != 0xFFFFFFFF), "ray leaked through the mesh"
 
 
-def _check_lbvh(os_, sc):
+def _check_lbvh(os_, sc, karras=False):
+    """karras: the scene was built without the treelet restructuring, so the
+    sorted ids are the stable Morton order itself; with it they are that
+    order's triangles renumbered depth first (a permutation)."""
     n = os_.tri_count
     keys, ids, kar = os_.export(2), os_.export(3), os_.export(4).reshape(-1, 4)
     nodes, stris = os_.export(0), os_.export(1)
@@ -115,12 +122,15 @@ def _check_lbvh(os_, sc):
         return out
     codes = (spread(q[:, 0]) << 2) | (spread(q[:, 1]) << 1) | spread(q[:, 2])
     order = np.argsort(codes, kind="stable")
-    assert np.array_equal(ids, order)
+    if karras:
+        assert np.array_equal(ids, order)
+    else:
+        assert np.array_equal(np.sort(ids), np.arange(n))
     assert np.array_equal(keys, codes[order])
     # sorted world triangles
     assert np.array_equal(stris["id"], ids)
     assert np.array_equal(stris["v0"], W[ids, 0])
-    # Karras: children cover the parent's contiguous range
+    # children cover the parent's contiguous range
     L = abi.LEAF_BIT
 
     def rng_of(c):
@@ -147,11 +157,13 @@ def _check_lbvh(os_, sc):
     return nodes
 
 
+@pytest.mark.parametrize("treelets", [True, False])
 @pytest.mark.parametrize("leaf_max", [1, 4, 8])
-def test_lbvh_invariants(oracle_lib, leaf_max):
+def test_lbvh_invariants(oracle_lib, leaf_max, treelets):
     sc = scenes.make_small(spheres=24)
-    os_ = oracle_lib.OracleScene(sc, leaf_max=leaf_max)
-    nodes = _check_lbvh(os_, sc)
+    with oracle_lib.treelets(treelets):
+        os_ = oracle_lib.OracleScene(sc, leaf_max=leaf_max)
+    nodes = _check_lbvh(os_, sc, karras=not treelets)
     # reachable leaves cover every triangle exactly once
     L = abi.LEAF_BIT
     covered = np.zeros(os_.tri_count, np.int64)
@@ -171,6 +183,107 @@ def test_lbvh_invariants(oracle_lib, leaf_max):
 def test_lbvh_invariants_terrain(oracle_lib):
     sc = scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=16, soup_tris=256)
     _check_lbvh(oracle_lib.OracleScene(sc), sc)
+    with oracle_lib.treelets(False):
+        os_ = oracle_lib.OracleScene(sc)
+    _check_lbvh(os_, sc, karras=True)
+
+
+SAH_NODE, SAH_TRI = 1.2, 1.0
+
+
+def _area(lo, hi):
+    d = np.asarray(hi, np.float64) - np.asarray(lo, np.float64)
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0]
+
+
+def _tree_cost(os_):
+    """SAH cost of the laid-out binary tree in float64: a node costs
+    SAH_NODE x its box area, a leaf range SAH_TRI x triangles x its area."""
+    nodes = os_.export(0)
+    L = abi.LEAF_BIT
+
+    def child(code, lo, hi):
+        if code & L:
+            return SAH_TRI * (((code >> 27) & 0xF) + 1) * _area(lo, hi)
+        return node(code)
+
+    def node(i):
+        nd = nodes[i]
+        lo = np.minimum(nd["lo0"], nd["lo1"])
+        hi = np.maximum(nd["hi0"], nd["hi1"])
+        return (SAH_NODE * _area(lo, hi) + child(int(nd["c0"]), nd["lo0"], nd["hi0"])
+                + child(int(nd["c1"]), nd["lo1"], nd["hi1"]))
+    return node(0)
+
+
+def test_treelets_lower_the_sah_cost(oracle_lib):
+    """Each treelet keeps its old topology when nothing is cheaper, so the
+    restructured tree never costs more than the Karras tree."""
+    for sc in (scenes.make_small(spheres=24),
+               scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=16, soup_tris=256)):
+        on = _tree_cost(oracle_lib.OracleScene(sc))
+        with oracle_lib.treelets(False):
+            off = _tree_cost(oracle_lib.OracleScene(sc))
+        assert on < off * 0.97, (on, off)
+
+
+def _soup_scene(k, seed, spread=4.0):
+    """k random triangles in one mesh"""
+    rng = np.random.default_rng(seed)
+    sc = _grid_scene(2)
+    V = np.zeros((3 * k, 11), np.float32)
+    c = rng.uniform(-spread, spread, size=(k, 1, 3))
+    V[:, 0:3] = (c + rng.uniform(-1, 1, size=(k, 3, 3))).reshape(-1, 3)
+    sc.meshes = [(V, np.arange(3 * k, dtype=np.uint32))]
+    sc.prims = scenes._prims([(0, 0, 3 * k, 0, np.eye(4))])
+    return sc
+
+
+def _best_cost(boxes, leaf_max):
+    """cheapest SAH cost over every binary tree with these triangle boxes as
+    leaves (exhaustive over the splits of every subset into two parts)"""
+    memo = {}
+
+    def best(S):
+        if S in memo:
+            return memo[S]
+        idx = [j for j in range(len(boxes)) if S >> j & 1]
+        lo = np.min([boxes[j][0] for j in idx], 0)
+        hi = np.max([boxes[j][1] for j in idx], 0)
+        A = _area(lo, hi)
+        if len(idx) <= leaf_max:
+            memo[S] = SAH_TRI * len(idx) * A
+            return memo[S]
+        c = min(best(P) + best(S ^ P) for P in range(1, S) if P & S == P)
+        memo[S] = SAH_NODE * A + c
+        return memo[S]
+    return best((1 << len(boxes)) - 1)
+
+
+@pytest.mark.parametrize("k,leaf_max", [(5, 1), (6, 2), (7, 2), (7, 1)])
+def test_treelet_tree_is_the_cheapest_on_one_treelet(oracle_lib, k, leaf_max):
+    for seed in range(6):
+        sc = _soup_scene(k, seed)
+        os_ = oracle_lib.OracleScene(sc, leaf_max=leaf_max)
+        st = os_.export(1)
+        boxes = [(np.min([t["v0"], t["v1"], t["v2"]], 0), np.max([t["v0"], t["v1"], t["v2"]], 0)) for t in st]
+        got, want = _tree_cost(os_), _best_cost(boxes, leaf_max)
+        assert got == pytest.approx(want, rel=1e-5), (seed, got, want)
+        with oracle_lib.treelets(False):
+            off = _tree_cost(oracle_lib.OracleScene(sc, leaf_max=leaf_max))
+        assert off >= want * (1 - 1e-5)
+
+
+def test_treelet_pin_catches_a_dropped_cost_term(oracle_lib):
+    worse = 0
+    with oracle_lib.planted(oracle_lib.planted.TREELET_COST):
+        for seed in range(6):
+            sc = _soup_scene(7, seed)
+            os_ = oracle_lib.OracleScene(sc, leaf_max=1)
+            st = os_.export(1)
+            boxes = [(np.min([t["v0"], t["v1"], t["v2"]], 0), np.max([t["v0"], t["v1"], t["v2"]], 0)) for t in st]
+            worse += _tree_cost(os_) > _best_cost(boxes, 1) * (1 + 1e-5)
+    assert worse > 0
 
 
 def test_lbvh_catches_planted_error(oracle_lib):

@@ -12,9 +12,12 @@
 //                  equal codes keep triangle-id order
 // 4. k_karras      Karras (HPG 2012) hierarchy from the sorted keys, ties
 //                  broken by the sorted index
-// 5. k_refit       bottom-up boxes, the second child to arrive builds the
-//                  parent (acquire/release arrival counters)
-// 6. k_layout      64-byte two-child-box nodes, subtrees of <= leaf_max
+// 5. k_refit_level bottom-up boxes, one launch per depth
+// 6. k_treelet_level  treelet restructuring (Karras and Aila, HPG 2013), one
+//                  launch per depth from the deepest; then the leaves are
+//                  renumbered depth first (k_relabel_*) so that every subtree
+//                  covers a contiguous range of sorted triangles again
+// 7. k_layout      64-byte two-child-box nodes, subtrees of <= leaf_max
 //                  triangles collapsed into leaf ranges; k_gather writes the
 //                  sorted 48-byte triangles
 // Every step is integer work or exact min/max, so the node and triangle arrays
@@ -386,6 +389,201 @@ __global__ __launch_bounds__(256) void k_refit_level(BuildBuffers b, uint32_t nn
     }
 }
 
+// ---- treelet restructuring (Karras and Aila, HPG 2013) ----------------------
+// One bottom-up pass over the Karras tree, a launch per depth from the
+// deepest (the nodes of one depth root disjoint subtrees, and everything
+// below them is done). Node N's treelet grows from N's two children by
+// replacing, while it has fewer than TREELET_LEAVES leaves, its internal
+// leaf of largest half box area (the first in list order on ties) by that
+// node's two children (the first in its place, the second appended). Over the
+// subsets S of the treelet leaves (bit j = leaf j), by increasing S:
+//   cost(S) = SAH_TRI * count(S) * area(S)                     count(S) <= leaf_max
+//           = SAH_NODE * area(S) + min_P cost(P) + cost(S \ P)   otherwise
+// with P over the subsets of S that hold S's lowest leaf (submasks in
+// decreasing order; the first minimum, ties to the split whose halves differ
+// least in leaf count). A treelet leaf costs its subtree's cost. The treelet
+// is rebuilt in its cheapest topology: N stays the root, the other internal
+// nodes are the ones it removed, in removal order, given out in pre-order.
+// area = dx*dy + dy*dz + dz*dx, boxes are unions in increasing leaf order.
+// Each restructured node's cost and triangle count go to the radix sort's
+// spare arrays (keys_tmp as float bits, ids_tmp), free after the sort.
+constexpr int TREELET_LEAVES = 7;
+constexpr float SAH_NODE = 1.2f, SAH_TRI = 1.0f;
+
+__device__ __forceinline__ float half_area(const float* lo, const float* hi) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* cost, uint32_t* cnt) {
+    const uint32_t leaf_max = b.leaf_max;
+    uint32_t tl[TREELET_LEAVES], freed[TREELET_LEAVES - 2];
+    int m = 2, nf = 0;
+    tl[0] = b.kl[N];
+    tl[1] = b.kr[N];
+    while (m < TREELET_LEAVES) {
+        int bi = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < m; ++i) {
+            if (tl[i] & LEAF_BIT) continue;
+            const float4 l = b.kbox[2 * (size_t)tl[i]], h = b.kbox[2 * (size_t)tl[i] + 1];
+            const float lo3[3] = {l.x, l.y, l.z}, hi3[3] = {h.x, h.y, h.z};
+            const float a = half_area(lo3, hi3);
+            if (a > ba) {
+                ba = a;
+                bi = i;
+            }
+        }
+        if (bi < 0) break;
+        const uint32_t x = tl[bi];
+        freed[nf++] = x;
+        tl[bi] = b.kl[x];
+        tl[m++] = b.kr[x];
+    }
+    const int ns = 1 << m;
+    float lo[TREELET_LEAVES][3], hi[TREELET_LEAVES][3];
+    uint32_t kc[TREELET_LEAVES];
+    float C[1 << TREELET_LEAVES];
+    uint8_t part[1 << TREELET_LEAVES];
+    for (int j = 0; j < m; ++j) {
+        float4 l, h;
+        child_box(b, tl[j], l, h);
+        lo[j][0] = l.x; lo[j][1] = l.y; lo[j][2] = l.z;
+        hi[j][0] = h.x; hi[j][1] = h.y; hi[j][2] = h.z;
+        const uint32_t k = (tl[j] & LEAF_BIT) ? 1u : cnt[tl[j]];
+        kc[j] = k;
+        C[1 << j] = ((tl[j] & LEAF_BIT) || k <= leaf_max) ? SAH_TRI * (float)k * half_area(lo[j], hi[j])
+                                                          : cost[tl[j]];
+        part[1 << j] = 0;
+    }
+    // union box and triangle count of a subset, members in increasing order
+    auto set_box = [&](int S, float* bl, float* bh) {
+        uint32_t k = 0;
+        bool firstm = true;
+        for (int j = 0; j < m; ++j) {
+            if (!((S >> j) & 1)) continue;
+            k += kc[j];
+            for (int q = 0; q < 3; ++q) {
+                bl[q] = firstm ? lo[j][q] : minsel(bl[q], lo[j][q]);
+                bh[q] = firstm ? hi[j][q] : maxsel(bh[q], hi[j][q]);
+            }
+            firstm = false;
+        }
+        return k;
+    };
+    for (int S = 1; S < ns; ++S) {
+        if ((S & (S - 1)) == 0) continue;
+        const int low = S & -S, pc = __popc((uint32_t)S);
+        float best = __uint_as_float(0x7f800000u);
+        int bp = 0, bbal = 1 << 30;
+        for (int P = (S - 1) & S; P > 0; P = (P - 1) & S) {
+            if (!(P & low)) continue;
+            const float c = C[P] + C[S ^ P];
+            const int d = 2 * __popc((uint32_t)P) - pc;
+            const int bal = d < 0 ? -d : d;
+            if (c < best || (c == best && bal < bbal)) {
+                best = c;
+                bp = P;
+                bbal = bal;
+            }
+        }
+        float bl[3], bh[3];
+        const uint32_t k = set_box(S, bl, bh);
+        const float A = half_area(bl, bh);
+        C[S] = k <= leaf_max ? SAH_TRI * (float)k * A : SAH_NODE * A + best;
+        part[S] = (uint8_t)bp;
+    }
+    // the internal subsets in pre-order, then their nodes
+    int order[TREELET_LEAVES - 1], stk[TREELET_LEAVES];
+    int no = 0, sp = 0;
+    stk[sp++] = ns - 1;
+    while (sp) {
+        const int S = stk[--sp];
+        order[no++] = S;
+        const int P = part[S] ? (int)part[S] : (S & -S), Q = S ^ P;
+        if (Q & (Q - 1)) stk[sp++] = Q;
+        if (P & (P - 1)) stk[sp++] = P;
+    }
+    auto code_of = [&](int S) -> uint32_t {
+        if ((S & (S - 1)) == 0) return tl[__builtin_ctz((uint32_t)S)];
+        for (int t = 0; t < no; ++t)
+            if (order[t] == S) return t == 0 ? N : freed[t - 1];
+        return 0xFFFFFFFFu;
+    };
+    for (int t = 0; t < no; ++t) {
+        const int S = order[t];
+        const int P = part[S] ? (int)part[S] : (S & -S), Q = S ^ P;
+        const uint32_t id = t == 0 ? N : freed[t - 1];
+        const uint32_t l = code_of(P), r = code_of(Q);
+        b.kl[id] = l;
+        b.kr[id] = r;
+        float bl[3], bh[3];
+        const uint32_t k = set_box(S, bl, bh);
+        b.kbox[2 * (size_t)id] = make_float4(bl[0], bl[1], bl[2], 0.0f);
+        b.kbox[2 * (size_t)id + 1] = make_float4(bh[0], bh[1], bh[2], 0.0f);
+        if (l & LEAF_BIT) b.lparent[l & ~LEAF_BIT] = id; else b.kparent[l] = id;
+        if (r & LEAF_BIT) b.lparent[r & ~LEAF_BIT] = id; else b.kparent[r] = id;
+        cost[id] = C[S];
+        cnt[id] = k;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_treelet_level(BuildBuffers b, uint32_t nn, uint32_t level) {
+    const uint32_t* off = b.lvl + nn;
+    const uint32_t first = off[level], last = off[level + 1u];
+    float* cost = reinterpret_cast<float*>(b.keys_tmp);
+    for (uint32_t t = first + blockIdx.x * blockDim.x + threadIdx.x; t < last; t += gridDim.x * blockDim.x)
+        restructure_treelet(b, b.lvl[t], cost, b.ids_tmp);
+}
+
+// Depth-first numbering (first child first) from the subtree triangle counts
+// (ids_tmp): a node's first leaf is the sum of the counts of the first
+// children along its path whose second child the path takes. Internal nodes
+// get their ranges, leaves their new positions (keys_tmp; the costs are dead).
+__global__ __launch_bounds__(256) void k_relabel_walk(BuildBuffers b, uint32_t n) {
+    const uint32_t nn = n - 1u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn + n) return;
+    const uint32_t* cnt = b.ids_tmp;
+    const uint32_t self = i < nn ? i : (LEAF_BIT | (i - nn));
+    uint32_t c = self, first = 0;
+    // a tree that passes the depth check is far shallower than the bound
+    for (uint32_t step = 0; c != 0u && step < 4096u; ++step) {
+        const uint32_t p = (c & LEAF_BIT) ? b.lparent[c & ~LEAF_BIT] : b.kparent[c];
+        if (b.kr[p] == c) {
+            const uint32_t l = b.kl[p];
+            first += (l & LEAF_BIT) ? 1u : cnt[l];
+        }
+        c = p;
+    }
+    if (i < nn) {
+        b.kfirst[i] = first;
+        b.klast[i] = first + cnt[i] - 1u;
+    } else {
+        b.keys_tmp[i - nn] = first;
+    }
+}
+
+// leaf codes and sorted ids in the new order (ids_tmp; the counts are dead)
+__global__ __launch_bounds__(256) void k_relabel_apply(BuildBuffers b, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t* pos = b.keys_tmp;
+    if (i + 1u < n) {
+        const uint32_t l = b.kl[i], r = b.kr[i];
+        if (l & LEAF_BIT) b.kl[i] = LEAF_BIT | pos[l & ~LEAF_BIT];
+        if (r & LEAF_BIT) b.kr[i] = LEAF_BIT | pos[r & ~LEAF_BIT];
+    }
+    if (i < n) b.ids_tmp[pos[i]] = b.ids[i];
+}
+
+__global__ __launch_bounds__(256) void k_leaf_parents(BuildBuffers b, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1u >= n) return;
+    const uint32_t l = b.kl[i], r = b.kr[i];
+    if (l & LEAF_BIT) b.lparent[l & ~LEAF_BIT] = i;
+    if (r & LEAF_BIT) b.lparent[r & ~LEAF_BIT] = i;
+}
+
 __global__ __launch_bounds__(256) void k_layout(BuildBuffers b) {
     uint32_t n = b.tri_count;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -647,7 +845,7 @@ __global__ __launch_bounds__(256) void k_depth(BuildBuffers b) {
     uint32_t d = 0;
     if (n >= 2) {
         uint32_t a = b.lparent[k];
-        for (;;) {
+        for (uint32_t step = 0; step < 4096u; ++step) {
             uint32_t sz = b.klast[a] - b.kfirst[a] + 1;
             if (a == 0 || sz > b.leaf_max) ++d;
             if (a == 0) break;
@@ -694,14 +892,30 @@ hipError_t refit_order(BuildBuffers& b, hipStream_t st) {
     return e;
 }
 
+void refit_boxes(BuildBuffers& b, hipStream_t st) {
+    const uint32_t nn = b.tri_count - 1u;
+    const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+    for (uint32_t L = b.refit_levels; L-- > 0u;) k_refit_level<<<grid, 256, 0, st>>>(b, nn, L);
+}
+
+// the treelet pass over the Karras tree (boxes and refit order in place),
+// the depth-first renumbering, and the refit order of the new topology
+hipError_t restructure(BuildBuffers& b, hipStream_t st) {
+    const uint32_t n = b.tri_count, nn = n - 1u;
+    const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+    for (uint32_t L = b.refit_levels; L-- > 0u;) k_treelet_level<<<grid, 256, 0, st>>>(b, nn, L);
+    k_relabel_walk<<<blocks(nn + n, 256), 256, 0, st>>>(b, n);
+    k_relabel_apply<<<blocks(n, 256), 256, 0, st>>>(b, n);
+    hipError_t e = hipMemcpyAsync(b.ids, b.ids_tmp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    k_leaf_parents<<<blocks(nn, 256), 256, 0, st>>>(b, n);
+    return refit_order(b, st);
+}
+
 // binary boxes bottom-up, the binary nodes, the sorted triangles and the depth
 hipError_t finish_binary(BuildBuffers& b, hipStream_t st) {
     uint32_t n = b.tri_count;
-    if (n >= 2) {
-        const uint32_t nn = n - 1u;
-        const uint32_t grid = std::min(blocks(nn, 256), 1024u);
-        for (uint32_t L = b.refit_levels; L-- > 0u;) k_refit_level<<<grid, 256, 0, st>>>(b, nn, L);
-    }
+    if (n >= 2) refit_boxes(b, st);
     uint32_t nn = n >= 2 ? n - 1 : 1;
     k_layout<<<blocks(nn, 256), 256, 0, st>>>(b);
     k_gather<<<blocks(n, 256), 256, 0, st>>>(b);
@@ -766,6 +980,11 @@ hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
     if (n >= 2) {
         k_karras<<<blocks(n - 1, 256), 256, 0, st>>>(b);
         e = refit_order(b, st);
+        if (e != hipSuccess) return e;
+    }
+    if (n >= 3) {
+        refit_boxes(b, st);
+        e = restructure(b, st);
         if (e != hipSuccess) return e;
     }
     e = finish_binary(b, st);
