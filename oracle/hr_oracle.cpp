@@ -12,7 +12,7 @@
  * plus the acceleration structure that replaces the driver's BLAS/TLAS
  * (renderer.cpp:1643-1916): a Karras (HPG 2012) LBVH over world-space
  * triangles with 30-bit Morton codes, stable key order, bottom-up boxes, one
- * pass of treelet restructuring (Karras and Aila, HPG 2013) and a
+ * two passes of treelet restructuring (Karras and Aila, HPG 2013) and a
  * two-child-box node layout, and a stack traversal with the watertight
  * ray/triangle test of Woop, Benthin and Wald (JCGT 2013).
  *
@@ -669,8 +669,8 @@ static void refit_boxes(or_scene& s) {
     }
 }
 /* ---- treelet restructuring (Karras and Aila, HPG 2013) ---------------------
- * One bottom-up pass over the Karras tree: every internal node N, after the
- * nodes below it, roots a treelet grown from N's two children by replacing,
+ * TREELET_PASSES bottom-up passes, the first over the Karras tree: every
+ * internal node N, after the nodes below it, roots a treelet grown from N's two children by replacing,
  * while it has fewer than TREELET_LEAVES leaves, its internal leaf of largest
  * half box area (the first in list order on ties) by that node's two
  * children (the first in its place, the second appended). Over the subsets S
@@ -687,7 +687,7 @@ static void refit_boxes(or_scene& s) {
  * boxes are unions in increasing leaf order. Then leaves are renumbered in
  * depth-first order (first child first), so every subtree again covers a
  * contiguous range of the sorted triangles; keys keep the Morton order. */
-constexpr int TREELET_LEAVES = 7;
+constexpr int TREELET_LEAVES = 7, TREELET_PASSES = 2;
 constexpr float SAH_NODE = 1.2f, SAH_TRI = 1.0f;
 static int g_treelets = 1;
 
@@ -792,20 +792,22 @@ static void restructure(or_scene& s) {
     if (n < 3) return;
     std::vector<float> cost(n - 1, 0.0f);
     std::vector<uint32_t> cnt(n - 1, 0u);
-    /* post-order: every node after the nodes below it */
-    std::vector<std::pair<uint32_t, int>> st;
-    st.push_back({0u, 0});
-    while (!st.empty()) {
-        auto& top = st.back();
-        const uint32_t N = top.first;
-        if (top.second == 0) {
-            top.second = 1;
-            if (!(s.kl[N] & LEAF_BIT)) st.push_back({s.kl[N], 0});
-            if (!(s.kr[N] & LEAF_BIT)) st.push_back({s.kr[N], 0});
-            continue;
+    for (int pass = 0; pass < TREELET_PASSES; ++pass) {
+        /* post-order: every node after the nodes below it */
+        std::vector<std::pair<uint32_t, int>> st;
+        st.push_back({0u, 0});
+        while (!st.empty()) {
+            auto& top = st.back();
+            const uint32_t N = top.first;
+            if (top.second == 0) {
+                top.second = 1;
+                if (!(s.kl[N] & LEAF_BIT)) st.push_back({s.kl[N], 0});
+                if (!(s.kr[N] & LEAF_BIT)) st.push_back({s.kr[N], 0});
+                continue;
+            }
+            st.pop_back();
+            restructure_treelet(s, N, cost, cnt);
         }
-        st.pop_back();
-        restructure_treelet(s, N, cost, cnt);
     }
     /* depth-first leaf order: new leaf positions and node ranges */
     std::vector<uint32_t> ids(n), newpos(n);

@@ -390,9 +390,10 @@ __global__ __launch_bounds__(256) void k_refit_level(BuildBuffers b, uint32_t nn
 }
 
 // ---- treelet restructuring (Karras and Aila, HPG 2013) ----------------------
-// One bottom-up pass over the Karras tree, a launch per depth from the
-// deepest (the nodes of one depth root disjoint subtrees, and everything
-// below them is done). Node N's treelet grows from N's two children by
+// TREELET_PASSES bottom-up passes, the first over the Karras tree, a launch
+// per depth from the deepest (the nodes of one depth root disjoint subtrees,
+// and everything below them is done; the depths are recomputed between
+// passes). Node N's treelet grows from N's two children by
 // replacing, while it has fewer than TREELET_LEAVES leaves, its internal
 // leaf of largest half box area (the first in list order on ties) by that
 // node's two children (the first in its place, the second appended). Over the
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(256) void k_refit_level(BuildBuffers b, uint32_t nn
 // area = dx*dy + dy*dz + dz*dx, boxes are unions in increasing leaf order.
 // Each restructured node's cost and triangle count go to the radix sort's
 // spare arrays (keys_tmp as float bits, ids_tmp), free after the sort.
-constexpr int TREELET_LEAVES = 7;
+constexpr int TREELET_LEAVES = 7, TREELET_PASSES = 2;
 constexpr float SAH_NODE = 1.2f, SAH_TRI = 1.0f;
 
 __device__ __forceinline__ float half_area(const float* lo, const float* hi) {
@@ -903,7 +904,13 @@ void refit_boxes(BuildBuffers& b, hipStream_t st) {
 hipError_t restructure(BuildBuffers& b, hipStream_t st) {
     const uint32_t n = b.tri_count, nn = n - 1u;
     const uint32_t grid = std::min(blocks(nn, 256), 1024u);
-    for (uint32_t L = b.refit_levels; L-- > 0u;) k_treelet_level<<<grid, 256, 0, st>>>(b, nn, L);
+    for (int pass = 0; pass < TREELET_PASSES; ++pass) {
+        if (pass > 0) {
+            hipError_t e = refit_order(b, st);  // the levels of the restructured tree
+            if (e != hipSuccess) return e;
+        }
+        for (uint32_t L = b.refit_levels; L-- > 0u;) k_treelet_level<<<grid, 256, 0, st>>>(b, nn, L);
+    }
     k_relabel_walk<<<blocks(nn + n, 256), 256, 0, st>>>(b, n);
     k_relabel_apply<<<blocks(n, 256), 256, 0, st>>>(b, n);
     hipError_t e = hipMemcpyAsync(b.ids, b.ids_tmp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
