@@ -416,54 +416,83 @@ __device__ __forceinline__ float half_area(const float* lo, const float* hi) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-__device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* cost, uint32_t* cnt) {
+// One lane per treelet. The subset tables (cost and chosen split per subset)
+// are the lane's column of two LDS arrays, [subset][lane]; the treelet's
+// leaves, boxes and counts stay in registers (loops unrolled to
+// TREELET_LEAVES with the leaf count as a guard).
+constexpr uint32_t TREELET_BLOCK = 64;
+constexpr int TREELET_SETS = 1 << TREELET_LEAVES;
+
+__device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* cost, uint32_t* cnt, float* C,
+                                    uint8_t* part) {
+    // C[S * TREELET_BLOCK], part[S * TREELET_BLOCK]: this lane's subset tables
     const uint32_t leaf_max = b.leaf_max;
     uint32_t tl[TREELET_LEAVES], freed[TREELET_LEAVES - 2];
-    int m = 2, nf = 0;
+#pragma unroll
+    for (int j = 0; j < TREELET_LEAVES; ++j) tl[j] = 0u;
+#pragma unroll
+    for (int j = 0; j < TREELET_LEAVES - 2; ++j) freed[j] = 0u;
+    int m = 2;
     tl[0] = b.kl[N];
     tl[1] = b.kr[N];
-    while (m < TREELET_LEAVES) {
+#pragma unroll
+    for (int step = 0; step < TREELET_LEAVES - 2; ++step) {
         int bi = -1;
         float ba = -1.0f;
-        for (int i = 0; i < m; ++i) {
-            if (tl[i] & LEAF_BIT) continue;
+#pragma unroll
+        for (int i = 0; i < TREELET_LEAVES; ++i) {
+            if (i >= m || (tl[i] & LEAF_BIT)) continue;
             const float4 l = b.kbox[2 * (size_t)tl[i]], h = b.kbox[2 * (size_t)tl[i] + 1];
             const float lo3[3] = {l.x, l.y, l.z}, hi3[3] = {h.x, h.y, h.z};
-            const float a = half_area(lo3, hi3);
-            if (a > ba) {
-                ba = a;
+            const float ar = half_area(lo3, hi3);
+            if (ar > ba) {
+                ba = ar;
                 bi = i;
             }
         }
         if (bi < 0) break;
-        const uint32_t x = tl[bi];
-        freed[nf++] = x;
-        tl[bi] = b.kl[x];
-        tl[m++] = b.kr[x];
+        uint32_t x = 0u;
+#pragma unroll
+        for (int i = 0; i < TREELET_LEAVES; ++i)
+            if (i == bi) x = tl[i];
+        const uint32_t xl = b.kl[x], xr = b.kr[x];
+        freed[step] = x;
+#pragma unroll
+        for (int i = 0; i < TREELET_LEAVES; ++i) {
+            if (i == bi) tl[i] = xl;
+            if (i == m) tl[i] = xr;
+        }
+        ++m;
     }
     const int ns = 1 << m;
     float lo[TREELET_LEAVES][3], hi[TREELET_LEAVES][3];
     uint32_t kc[TREELET_LEAVES];
-    float C[1 << TREELET_LEAVES];
-    uint8_t part[1 << TREELET_LEAVES];
-    for (int j = 0; j < m; ++j) {
+#pragma unroll
+    for (int j = 0; j < TREELET_LEAVES; ++j) {
+        kc[j] = 0u;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) lo[j][q] = hi[j][q] = 0.0f;
+        if (j >= m) continue;
         float4 l, h;
         child_box(b, tl[j], l, h);
         lo[j][0] = l.x; lo[j][1] = l.y; lo[j][2] = l.z;
         hi[j][0] = h.x; hi[j][1] = h.y; hi[j][2] = h.z;
         const uint32_t k = (tl[j] & LEAF_BIT) ? 1u : cnt[tl[j]];
         kc[j] = k;
-        C[1 << j] = ((tl[j] & LEAF_BIT) || k <= leaf_max) ? SAH_TRI * (float)k * half_area(lo[j], hi[j])
-                                                          : cost[tl[j]];
-        part[1 << j] = 0;
+        C[(1 << j) * TREELET_BLOCK] = ((tl[j] & LEAF_BIT) || k <= leaf_max)
+                                          ? SAH_TRI * (float)k * half_area(lo[j], hi[j])
+                                          : cost[tl[j]];
+        part[(1 << j) * TREELET_BLOCK] = 0;
     }
     // union box and triangle count of a subset, members in increasing order
     auto set_box = [&](int S, float* bl, float* bh) {
         uint32_t k = 0;
         bool firstm = true;
-        for (int j = 0; j < m; ++j) {
-            if (!((S >> j) & 1)) continue;
+#pragma unroll
+        for (int j = 0; j < TREELET_LEAVES; ++j) {
+            if (j >= m || !((S >> j) & 1)) continue;
             k += kc[j];
+#pragma unroll
             for (int q = 0; q < 3; ++q) {
                 bl[q] = firstm ? lo[j][q] : minsel(bl[q], lo[j][q]);
                 bh[q] = firstm ? hi[j][q] : maxsel(bh[q], hi[j][q]);
@@ -479,7 +508,7 @@ __device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* co
         int bp = 0, bbal = 1 << 30;
         for (int P = (S - 1) & S; P > 0; P = (P - 1) & S) {
             if (!(P & low)) continue;
-            const float c = C[P] + C[S ^ P];
+            const float c = C[P * TREELET_BLOCK] + C[(S ^ P) * TREELET_BLOCK];
             const int d = 2 * __popc((uint32_t)P) - pc;
             const int bal = d < 0 ? -d : d;
             if (c < best || (c == best && bal < bbal)) {
@@ -491,30 +520,51 @@ __device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* co
         float bl[3], bh[3];
         const uint32_t k = set_box(S, bl, bh);
         const float A = half_area(bl, bh);
-        C[S] = k <= leaf_max ? SAH_TRI * (float)k * A : SAH_NODE * A + best;
-        part[S] = (uint8_t)bp;
+        C[S * TREELET_BLOCK] = k <= leaf_max ? SAH_TRI * (float)k * A : SAH_NODE * A + best;
+        part[S * TREELET_BLOCK] = (uint8_t)bp;
     }
     // the internal subsets in pre-order, then their nodes
     int order[TREELET_LEAVES - 1], stk[TREELET_LEAVES];
+#pragma unroll
+    for (int j = 0; j < TREELET_LEAVES - 1; ++j) order[j] = 0;
     int no = 0, sp = 0;
     stk[sp++] = ns - 1;
     while (sp) {
         const int S = stk[--sp];
         order[no++] = S;
-        const int P = part[S] ? (int)part[S] : (S & -S), Q = S ^ P;
+        const int ps = (int)part[S * TREELET_BLOCK];
+        const int P = ps ? ps : (S & -S), Q = S ^ P;
         if (Q & (Q - 1)) stk[sp++] = Q;
         if (P & (P - 1)) stk[sp++] = P;
     }
     auto code_of = [&](int S) -> uint32_t {
-        if ((S & (S - 1)) == 0) return tl[__builtin_ctz((uint32_t)S)];
-        for (int t = 0; t < no; ++t)
-            if (order[t] == S) return t == 0 ? N : freed[t - 1];
-        return 0xFFFFFFFFu;
+        uint32_t code = 0xFFFFFFFFu;
+        if ((S & (S - 1)) == 0) {
+#pragma unroll
+            for (int j = 0; j < TREELET_LEAVES; ++j)
+                if (S == (1 << j)) code = tl[j];
+            return code;
+        }
+#pragma unroll
+        for (int t = 0; t < TREELET_LEAVES - 1; ++t) {
+            if (t >= no || order[t] != S) continue;
+            code = N;
+#pragma unroll
+            for (int f = 0; f < TREELET_LEAVES - 2; ++f)
+                if (t == f + 1) code = freed[f];
+        }
+        return code;
     };
-    for (int t = 0; t < no; ++t) {
+#pragma unroll
+    for (int t = 0; t < TREELET_LEAVES - 1; ++t) {
+        if (t >= no) continue;
         const int S = order[t];
-        const int P = part[S] ? (int)part[S] : (S & -S), Q = S ^ P;
-        const uint32_t id = t == 0 ? N : freed[t - 1];
+        const int ps = (int)part[S * TREELET_BLOCK];
+        const int P = ps ? ps : (S & -S), Q = S ^ P;
+        uint32_t id = N;
+#pragma unroll
+        for (int f = 0; f < TREELET_LEAVES - 2; ++f)
+            if (t == f + 1) id = freed[f];
         const uint32_t l = code_of(P), r = code_of(Q);
         b.kl[id] = l;
         b.kr[id] = r;
@@ -524,17 +574,19 @@ __device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* co
         b.kbox[2 * (size_t)id + 1] = make_float4(bh[0], bh[1], bh[2], 0.0f);
         if (l & LEAF_BIT) b.lparent[l & ~LEAF_BIT] = id; else b.kparent[l] = id;
         if (r & LEAF_BIT) b.lparent[r & ~LEAF_BIT] = id; else b.kparent[r] = id;
-        cost[id] = C[S];
+        cost[id] = C[S * TREELET_BLOCK];
         cnt[id] = k;
     }
 }
 
-__global__ __launch_bounds__(256) void k_treelet_level(BuildBuffers b, uint32_t nn, uint32_t level) {
+__global__ __launch_bounds__(TREELET_BLOCK) void k_treelet_level(BuildBuffers b, uint32_t nn, uint32_t level) {
+    __shared__ float sC[TREELET_SETS * TREELET_BLOCK];
+    __shared__ uint8_t sPart[TREELET_SETS * TREELET_BLOCK];
     const uint32_t* off = b.lvl + nn;
     const uint32_t first = off[level], last = off[level + 1u];
     float* cost = reinterpret_cast<float*>(b.keys_tmp);
     for (uint32_t t = first + blockIdx.x * blockDim.x + threadIdx.x; t < last; t += gridDim.x * blockDim.x)
-        restructure_treelet(b, b.lvl[t], cost, b.ids_tmp);
+        restructure_treelet(b, b.lvl[t], cost, b.ids_tmp, sC + threadIdx.x, sPart + threadIdx.x);
 }
 
 // Depth-first numbering (first child first) from the subtree triangle counts
@@ -903,13 +955,13 @@ void refit_boxes(BuildBuffers& b, hipStream_t st) {
 // the depth-first renumbering, and the refit order of the new topology
 hipError_t restructure(BuildBuffers& b, hipStream_t st) {
     const uint32_t n = b.tri_count, nn = n - 1u;
-    const uint32_t grid = std::min(blocks(nn, 256), 1024u);
+    const uint32_t tgrid = std::min(blocks(nn, TREELET_BLOCK), 4096u);
     for (int pass = 0; pass < TREELET_PASSES; ++pass) {
         if (pass > 0) {
             hipError_t e = refit_order(b, st);  // the levels of the restructured tree
             if (e != hipSuccess) return e;
         }
-        for (uint32_t L = b.refit_levels; L-- > 0u;) k_treelet_level<<<grid, 256, 0, st>>>(b, nn, L);
+        for (uint32_t L = b.refit_levels; L-- > 0u;) k_treelet_level<<<tgrid, TREELET_BLOCK, 0, st>>>(b, nn, L);
     }
     k_relabel_walk<<<blocks(nn + n, 256), 256, 0, st>>>(b, n);
     k_relabel_apply<<<blocks(n, 256), 256, 0, st>>>(b, n);
