@@ -556,10 +556,22 @@ __global__ void k_features(FeatArgs a, uint32_t n) {
 
 // kernel prediction: out = sum_k softmax(logits)_k * noisy[clamp(p + off_k)]
 // over the 5x5 neighbourhood, off_k = (k % 5 - 2, k / 5 - 2)
+// lg: a pixel's row of 32 logits (128-byte aligned), read as 16-byte chunks
 __device__ __forceinline__ void softmax25(const float* lg, float* w) {
+    const float4* l4 = reinterpret_cast<const float4*>(lg);
+    float v[28];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const float4 t = l4[q];
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
+    }
     float m = -INFINITY;
+#pragma unroll
     for (int k = 0; k < 25; ++k) {
-        w[k] = lg[k];
+        w[k] = v[k];
         m = fmaxf(m, w[k]);
     }
     float s = 0.0f;
