@@ -430,12 +430,20 @@ void hr_graph_destroy(hr_graph* graph);
  *                  slab). The workspace is allocated when a frame first needs
  *                  it and kept by the context; when the device cannot hold it,
  *                  the budget is halved (down to 64 MiB) and the frame runs in
- *                  more, smaller slabs, with the same results. */
+ *                  more, smaller slabs, with the same results.
+ *  bvh_flags       how scenes created afterwards build their hierarchy: 0 (the
+ *                  default) restructures the LBVH by treelets (Karras and Aila
+ *                  2013: two passes of 7-leaf treelets, fewer node visits per
+ *                  ray for a longer build); HR_BVH_NO_TREELETS keeps the plain
+ *                  Karras tree. A restructured tree deeper than the traversal
+ *                  stack (64 levels) is replaced by the plain tree. Other bits:
+ *                  HR_E_INVALID. Hits do not depend on the hierarchy. */
+#define HR_BVH_NO_TREELETS 1u
 typedef struct hr_tuning {
     uint32_t blocks_per_cu;
     uint32_t tile_w;
     uint32_t wave_budget_mb;
-    uint32_t _reserved;
+    uint32_t bvh_flags;
 } hr_tuning;
 hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* tuning);
 hr_status hr_context_get_tuning(const hr_context* ctx, hr_tuning* tuning);

@@ -678,8 +678,8 @@ static void refit_boxes(or_scene& s) {
  *   cost(S) = SAH_TRI * count(S) * area(S)          if count(S) <= leaf_max
  *           = SAH_NODE * area(S) + min_P cost(P) + cost(S \ P)   otherwise
  * with P over the subsets of S that hold S's lowest leaf (submasks in
- * decreasing order, the first minimum, ties to the split whose halves differ
- * least in leaf count); a single leaf costs its subtree's cost (a triangle or
+ * decreasing order, the first minimum, ties to the split whose parts differ
+ * least in triangle count); a single leaf costs its subtree's cost (a triangle or
  * a subtree of at most leaf_max triangles: SAH_TRI * count * area). The
  * treelet is rebuilt in its minimum-cost topology: N keeps the root, the other
  * internal nodes are the removed ones in removal order, assigned in pre-order
@@ -740,15 +740,20 @@ static void restructure_treelet(or_scene& s, uint32_t N, std::vector<float>& cos
     };
     for (int S = 1; S < ns; ++S) {
         if ((S & (S - 1)) == 0) continue;
-        const int low = S & -S, pc = __builtin_popcount((unsigned)S);
+        const int low = S & -S;
         K[S] = K[low] + K[S ^ low];
         float best = INFINITY;
-        int bp = 0, bbal = 1 << 30;
+        int bp = 0;
+        /* ties: the split whose parts differ least in triangle count */
+        auto imbalance = [&](int P) -> uint32_t {
+            if (P == 0) return 0xFFFFFFFFu;
+            const uint32_t a = K[P], b = K[S ^ P];
+            return a > b ? a - b : b - a;
+        };
         for (int P = (S - 1) & S; P > 0; P = (P - 1) & S) {
             if (!(P & low)) continue;
             const float c = g_plant == PLANT_TREELET_COST ? C[P] : C[P] + C[S ^ P];
-            const int bal = std::abs(2 * __builtin_popcount((unsigned)P) - pc);
-            if (c < best || (c == best && bal < bbal)) { best = c; bp = P; bbal = bal; }
+            if (c < best || (c == best && imbalance(P) < imbalance(bp))) { best = c; bp = P; }
         }
         float b[6];
         set_box(S, b);
@@ -931,7 +936,12 @@ static void compute_depth(or_scene& s) {
     s.depth = best;
 }
 
-static void build_bvh(or_scene& s) {
+/* the GPU library's traversal stack bound (internal.h MAX_DEPTH) */
+constexpr uint32_t TRAVERSAL_DEPTH_MAX = 64;
+
+/* treelets: restructure the Karras tree; a restructured tree deeper than
+ * TRAVERSAL_DEPTH_MAX is rebuilt without (the treelet cost ignores depth) */
+static void build_bvh(or_scene& s, bool treelets) {
     uint32_t n = s.ntri;
     s.keys.assign(n, 0);
     s.ids.resize(n);
@@ -964,10 +974,11 @@ static void build_bvh(or_scene& s) {
     if (n >= 2) {
         karras(s);
         refit_boxes(s);
-        if (g_treelets) restructure(s);
+        if (treelets) restructure(s);
     }
     layout_nodes(s);
     compute_depth(s);
+    if (treelets && n >= 3 && s.depth > TRAVERSAL_DEPTH_MAX) build_bvh(s, false);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1618,7 +1629,7 @@ int or_scene_build(const hr_mesh* meshes, uint32_t mesh_count, const hr_primitiv
     s->wtri.assign((size_t)tri * 9, 0.0f);
     s->tbox.assign((size_t)tri * 6, 0.0f);
     transform_tris(*s);
-    build_bvh(*s);
+    build_bvh(*s, g_treelets != 0);
     *out = s;
     return HR_OK;
 }

@@ -84,6 +84,42 @@ def test_lbvh_bit_exact_terrain(gpu_ctx, oracle_lib):
         assert np.array_equal(_bits(hs.export(which)), _bits(os_.export(which))), which
 
 
+def test_lbvh_bit_exact_without_treelets(gpu_ctx, oracle_lib):
+    """HR_BVH_NO_TREELETS (the plain Karras tree, also what a too-deep
+    restructured tree falls back to) against the oracle built the same way"""
+    sc = scenes.make_terrain_scene(seed=3, width=64, height=36, quads=128, instances=32, soup_tris=512)
+    gpu_ctx.set_bvh_flags(abi.HR_BVH_NO_TREELETS)
+    try:
+        hs = _hr().HybridScene(gpu_ctx, sc, leaf_max=2)
+    finally:
+        gpu_ctx.set_bvh_flags(0)
+    with oracle_lib.treelets(False):
+        os_ = oracle_lib.OracleScene(sc, leaf_max=2)
+    assert hs.info()["depth"] == os_.depth
+    for which in (2, 3, 4, 0, 1):
+        assert np.array_equal(_bits(hs.export(which)), _bits(os_.export(which))), which
+    # and the restructured tree of the same scene is a different one
+    assert not np.array_equal(_bits(_hr().HybridScene(gpu_ctx, sc, leaf_max=2).export(4)), _bits(os_.export(4)))
+
+
+def test_lbvh_bit_exact_coincident(gpu_ctx, oracle_lib):
+    """3000 copies of one triangle: every treelet split ties"""
+    from test_oracle_geometry import coincident_scene
+    sc = coincident_scene(3000)
+    hs, os_ = _pair(gpu_ctx, oracle_lib, sc, leaf_max=2)
+    assert hs.info()["depth"] == os_.depth <= 64
+    for which in (2, 3, 4, 0, 1):
+        assert np.array_equal(_bits(hs.export(which)), _bits(os_.export(which))), which
+
+
+def test_unknown_bvh_flags_are_rejected(gpu_ctx):
+    hr = _hr()
+    with pytest.raises(hr.HRError) as e:
+        gpu_ctx.set_bvh_flags(2)
+    assert e.value.status == abi.HR_E_INVALID
+    assert gpu_ctx.bvh_flags() == 0
+
+
 def test_lbvh_refit_bit_exact(gpu_ctx, oracle_lib):
     sc = scenes.make_small(spheres=16)
     hs, os_ = _pair(gpu_ctx, oracle_lib, sc)

@@ -112,7 +112,8 @@ def _check_lbvh(os_, sc, karras=False):
     cmin, cmax = c.min(0), c.max(0)
     ext = cmax - cmin
     # one cell size for the three axes: 1024 cells along the longest extent
-    scale = np.where(ext > 0, np.float32(1024) / ext.max(), 0).astype(np.float32)
+    em = ext.max() if ext.max() > 0 else np.float32(1)
+    scale = np.where(ext > 0, np.float32(1024) / em, 0).astype(np.float32)
     q = np.clip(np.floor((c - cmin) * scale), 0, 1023).astype(np.uint32)
 
     def spread(v):
@@ -272,6 +273,37 @@ def test_treelet_tree_is_the_cheapest_on_one_treelet(oracle_lib, k, leaf_max):
         with oracle_lib.treelets(False):
             off = _tree_cost(oracle_lib.OracleScene(sc, leaf_max=leaf_max))
         assert off >= want * (1 - 1e-5)
+
+
+def coincident_scene(k):
+    """k copies of one triangle (every box and centroid equal)"""
+    sc = _grid_scene(2)
+    V = np.zeros((3 * k, 11), np.float32)
+    V[:, 0:3] = np.tile(np.array([(-1, -1, 2), (1, -1, 2), (0, 1, 2)], np.float32), (k, 1))
+    sc.meshes = [(V, np.arange(3 * k, dtype=np.uint32))]
+    sc.prims = scenes._prims([(0, 0, 3 * k, 0, np.eye(4))])
+    return sc
+
+
+def test_treelets_on_coincident_triangles(oracle_lib):
+    """Every split ties: the tie rule (even triangle counts) keeps the tree
+    within the traversal stack, the structure stays valid and hits are those
+    of a brute-force scan (the smallest id of equal t)."""
+    sc = coincident_scene(3000)
+    os_ = oracle_lib.OracleScene(sc, leaf_max=2)
+    assert os_.depth <= 64
+    _check_lbvh(os_, sc)
+    rays = np.zeros(64, dtype=abi.RAY_DTYPE)
+    rng = np.random.default_rng(1)
+    rays["org"][:, 0:2] = rng.uniform(-0.4, 0.4, size=(64, 2))
+    rays["dir"] = (0, 0, 1)
+    rays["tmin"], rays["tmax"] = 1e-3, 10
+    for any_hit in (False, True):
+        b, _ = os_.trace_rays(rays, any_hit=any_hit, brute=True)
+        t, _ = os_.trace_rays(rays, any_hit=any_hit, brute=False)
+        assert np.all(t["tri"] != 0xFFFFFFFF)
+        if not any_hit:
+            assert np.array_equal(b["tri"], t["tri"]) and np.all(t["tri"] == 0) and np.all(t["t"] == 2.0)
 
 
 def test_treelet_pin_catches_a_dropped_cost_term(oracle_lib):

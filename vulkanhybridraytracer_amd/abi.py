@@ -19,6 +19,7 @@ HR_LIGHT_DISK, HR_LIGHT_RECT = 0, 1
 HR_FLAG_COUNT_VISITS, HR_FLAG_TIMING, HR_FLAG_ADAPTIVE = 1, 2, 4
 HR_FLAG_SHADOW_R8, HR_FLAG_LIT_BGRA8 = 8, 16  # the reference's R8_UNORM shadow / B8G8R8A8_UNORM lit images
 HR_TRACE_ANY_HIT, HR_TRACE_COUNT_VISITS = 1, 2  # hr_trace_rays flags
+HR_BVH_NO_TREELETS = 1  # hr_tuning.bvh_flags: the plain Karras tree
 
 
 class hr_vertex(C.Structure):
@@ -126,7 +127,7 @@ class hr_frame_desc(C.Structure):
 
 class hr_tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("tile_w", C.c_uint32), ("wave_budget_mb", C.c_uint32),
-                ("_reserved", C.c_uint32)]
+                ("bvh_flags", C.c_uint32)]
 
 
 for _s, _n in ((hr_vertex, 44), (hr_material, 48), (hr_light, 48), (hr_light_shape, 32),

@@ -773,7 +773,7 @@ hr_status hr_scene_build(hr_context* ctx, const hr_mesh* meshes, uint32_t mesh_c
     b.leaf_max = leaf_max;
 
     he = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (he == hipSuccess) he = lbvh_build(b, ctx->stream);
+    if (he == hipSuccess) he = lbvh_build(b, ctx->stream, !(ctx->tuning.bvh_flags & HR_BVH_NO_TREELETS));
     if (he == hipSuccess && n > 0) {  // the 4-wide nodes, dense: as many as the build chose
         st = HR_OK;
         check(scene_alloc(s, &b.nodes4, b.wide_count));
@@ -1127,8 +1127,8 @@ hr_status hr_context_set_tuning(hr_context* ctx, const hr_tuning* t) {
     bool pow2 = t->tile_w && (t->tile_w & (t->tile_w - 1)) == 0;
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || !pow2 || t->tile_w > 64 || t->wave_budget_mb < 64)
         return fail(ctx, HR_E_INVALID, "tuning out of range");
+    if (t->bvh_flags & ~HR_BVH_NO_TREELETS) return fail(ctx, HR_E_INVALID, "unknown bvh_flags 0x%x", t->bvh_flags);
     ctx->tuning = *t;
-    ctx->tuning._reserved = 0;
     ctx->resident_blocks = ctx->cu_count * t->blocks_per_cu;
     return HR_OK;
 }

@@ -267,7 +267,9 @@ size_t radix_hist_words(uint32_t n);
 // full build in two calls: lbvh_build makes the binary tree and chooses the
 // 4-wide nodes (sets wide_count, synchronising the stream); the caller then
 // allocates nodes4 and wsrc with wide_count entries, and lbvh_build_wide fills them
-hipError_t lbvh_build(BuildBuffers& b, hipStream_t st);
+// treelets: restructure the Karras tree (kept only if the result passes the
+// traversal depth bound MAX_DEPTH; else the plain tree is rebuilt)
+hipError_t lbvh_build(BuildBuffers& b, hipStream_t st, bool treelets);
 hipError_t lbvh_build_wide(BuildBuffers& b, hipStream_t st);
 hipError_t lbvh_refit(BuildBuffers& b, hipStream_t st);   // new models, same topology (binary and 4-wide)
 

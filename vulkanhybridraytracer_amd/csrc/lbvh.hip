@@ -401,8 +401,8 @@ __global__ __launch_bounds__(256) void k_refit_level(BuildBuffers b, uint32_t nn
 //   cost(S) = SAH_TRI * count(S) * area(S)                     count(S) <= leaf_max
 //           = SAH_NODE * area(S) + min_P cost(P) + cost(S \ P)   otherwise
 // with P over the subsets of S that hold S's lowest leaf (submasks in
-// decreasing order; the first minimum, ties to the split whose halves differ
-// least in leaf count). A treelet leaf costs its subtree's cost. The treelet
+// decreasing order; the first minimum, ties to the split whose parts differ
+// least in triangle count). A treelet leaf costs its subtree's cost. The treelet
 // is rebuilt in its cheapest topology: N stays the root, the other internal
 // nodes are the ones it removed, in removal order, given out in pre-order.
 // area = dx*dy + dy*dz + dz*dx, boxes are unions in increasing leaf order.
@@ -501,20 +501,30 @@ __device__ void restructure_treelet(const BuildBuffers& b, uint32_t N, float* co
         }
         return k;
     };
+    auto tri_count = [&](int X) {
+        uint32_t k = 0;
+#pragma unroll
+        for (int j = 0; j < TREELET_LEAVES; ++j)
+            if (j < m && ((X >> j) & 1)) k += kc[j];
+        return k;
+    };
     for (int S = 1; S < ns; ++S) {
         if ((S & (S - 1)) == 0) continue;
-        const int low = S & -S, pc = __popc((uint32_t)S);
+        const int low = S & -S;
         float best = __uint_as_float(0x7f800000u);
-        int bp = 0, bbal = 1 << 30;
+        int bp = 0;
+        // ties: the split whose parts differ least in triangle count
+        auto imbalance = [&](int P) -> uint32_t {
+            if (P == 0) return 0xFFFFFFFFu;
+            const uint32_t a = tri_count(P), b = tri_count(S ^ P);
+            return a > b ? a - b : b - a;
+        };
         for (int P = (S - 1) & S; P > 0; P = (P - 1) & S) {
             if (!(P & low)) continue;
             const float c = C[P * TREELET_BLOCK] + C[(S ^ P) * TREELET_BLOCK];
-            const int d = 2 * __popc((uint32_t)P) - pc;
-            const int bal = d < 0 ? -d : d;
-            if (c < best || (c == best && bal < bbal)) {
+            if (c < best || (c == best && imbalance(P) < imbalance(bp))) {
                 best = c;
                 bp = P;
-                bbal = bal;
             }
         }
         float bl[3], bh[3];
@@ -1027,7 +1037,7 @@ hipError_t choose_wide(BuildBuffers& b, hipStream_t st) {
 
 size_t radix_hist_words(uint32_t n) { return 256u * (size_t)blocks(n > 0 ? n : 1, RS_TILE); }
 
-hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
+hipError_t lbvh_build(BuildBuffers& b, hipStream_t st, bool treelets) {
     uint32_t n = b.tri_count;
     b.wide_count = 0;
     if (n == 0) return hipSuccess;
@@ -1041,13 +1051,24 @@ hipError_t lbvh_build(BuildBuffers& b, hipStream_t st) {
         e = refit_order(b, st);
         if (e != hipSuccess) return e;
     }
-    if (n >= 3) {
+    const bool restructured = treelets && n >= 3;
+    if (restructured) {
         refit_boxes(b, st);
         e = restructure(b, st);
         if (e != hipSuccess) return e;
     }
     e = finish_binary(b, st);
     if (e != hipSuccess) return e;
+    if (restructured) {
+        // the treelet cost ignores depth: a tree too deep for the traversal
+        // stack is rebuilt without it (the plain tree's own depth check
+        // follows in the caller)
+        uint32_t depth = 0;
+        e = hipMemcpyAsync(&depth, b.depth, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        if (depth > MAX_DEPTH) return lbvh_build(b, st, false);
+    }
     return choose_wide(b, st);
 }
 

@@ -184,14 +184,28 @@ class Context:
         self.check(self.L.hr_post_gamma(self.h, width, height, src.ptr, None if out is None else out.ptr,
                                         None if rgba8 is None else rgba8.ptr))
 
+    def _tuning(self):
+        t = abi.hr_tuning()
+        self.check(self.L.hr_context_get_tuning(self.h, C.byref(t)))
+        return t
+
     def set_tuning(self, blocks_per_cu=8, tile_w=8, wave_budget_mb=40960):
-        t = abi.hr_tuning(blocks_per_cu, tile_w, wave_budget_mb, 0)
+        """launch shapes (the hierarchy flags are kept)"""
+        t = abi.hr_tuning(blocks_per_cu, tile_w, wave_budget_mb, self._tuning().bvh_flags)
         self.check(self.L.hr_context_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
-        t = abi.hr_tuning()
-        self.check(self.L.hr_context_get_tuning(self.h, C.byref(t)))
+        t = self._tuning()
         return dict(blocks_per_cu=t.blocks_per_cu, tile_w=t.tile_w, wave_budget_mb=t.wave_budget_mb)
+
+    def set_bvh_flags(self, flags):
+        """HR_BVH_* flags of the scenes created afterwards"""
+        t = self._tuning()
+        t.bvh_flags = flags
+        self.check(self.L.hr_context_set_tuning(self.h, C.byref(t)))
+
+    def bvh_flags(self):
+        return self._tuning().bvh_flags
 
 
 class DeviceBuffer:
