@@ -317,8 +317,9 @@ hr_status hr_scene_info(const hr_scene* scene, uint32_t* tri_count, uint32_t* no
 hr_status hr_scene_memory(const hr_scene* scene, uint32_t* wide_count, uint64_t* device_bytes);
 /* which: 0 = traversal nodes (node_count x 64 B), 1 = sorted world-space
  * triangles (tri_count x 48 B), 2 = Morton codes in sorted order (tri_count x
- * 4 B), 3 = sorted triangle ids (tri_count x 4 B), 4 = Karras internal nodes
- * (left, right, first, last) ((tri_count - 1) x 16 B), 5 = 4-wide traversal
+ * 4 B), 3 = sorted triangle ids, in the order of the sorted triangles
+ * (tri_count x 4 B), 4 = internal nodes of the binary tree (left, right,
+ * first, last) ((tri_count - 1) x 16 B), 5 = 4-wide traversal
  * nodes (wide_count x 128 B of hr_scene_memory, the root first: from the root
  * down, a wide node takes its binary node's two children and twice replaces
  * the internal child of largest box area, (x y + y z) + z x of the extents,
