@@ -508,16 +508,20 @@ __global__ __launch_bounds__(256) void k_store_job(ShadowJob job, uint32_t* dst)
 inline uint32_t pixel_blocks(const FrameArgs& f) { return ((f.width + 15u) / 16u) * ((f.height + 15u) / 16u); }
 
 
-// Sample-parallel shadow pass for one frame without adaptive sampling and a
-// power-of-two sample count S in [4, 64]: the work items are (group, sample)
-// pairs, numbered so a group's S samples are consecutive, and a lane traces
-// one item at a time. Lanes that take work together take consecutive items,
-// i.e. the nearly parallel rays of one pixel towards one light, which keeps a
-// wave's rays coherent. An unoccluded sample sets bit `item` of a bit array
-// (atomicOr: the result does not depend on the order); k_shadow_bits turns
-// each group's bits into the factor count / S. Items are fetched from the
-// same per-XCD bands as in k_shadows.
-// MODE 0: S samples per group (no adaptive sampling). With adaptive sampling
+// Sample-parallel shadow pass for a power-of-two sample count S in [4, 64]:
+// the work items are (group, sample) pairs, numbered so a group's S samples
+// are consecutive, and a lane traces one item at a time. Lanes that take work
+// together take consecutive items, i.e. the nearly parallel rays of one pixel
+// towards one light, which keeps a wave's rays coherent. An unoccluded sample
+// sets bit (group << lg2s) | sample of a bit array (atomicOr: the result does
+// not depend on the order); k_shadow_bits turns each group's bits into the
+// factor count / S. Items are fetched from the same per-XCD bands as in
+// k_shadows.
+// Without adaptive sampling, k_group_kinds / k_tile_totals / k_scan_counts /
+// k_tile_offsets / k_group_scatter first settle the groups that need no ray
+// and list the others in order, and MODE 3 traces the S samples of each
+// listed group (item = list slot * 2^lg2s + sample; the item count is read
+// from *list_count on the device). With adaptive sampling
 // (the nested sequence of adaptive_stop / adaptive_next) each group owns a
 // window of 2^lg2s bits, bit s for sample s, that every stage adds to: MODE 1
 // traces the probes (items numbered 4 per group; 1 probe for a hard light),
@@ -538,6 +542,47 @@ __device__ __forceinline__ uint32_t job_of(const ShadowJob* __restrict__ jobs, u
     return lo;
 }
 
+// The (pixel, light) part of a shadow item, the same for all its samples:
+// group g of job J (tile t, light li, pixel q of the tile). ITEM_SKIP (factor
+// 0: outside the frame, background, the light behind the surface), ITEM_LIT
+// (factor 1: glass, beyond the light's reach) or ITEM_TRACE with the surface
+// point, the unit vector to the light and its distance.
+struct GroupRay {
+    v3 P, L;
+    float dist;
+    bool isDir;
+    uint32_t x, y;
+};
+__device__ __forceinline__ int group_kind(const ShadowJob& J, uint32_t t, uint32_t li, uint32_t q, GroupRay& o) {
+    const FrameArgs& F = J.F;
+    uint32_t x, y;
+    tile_pixel(J, t, q, x, y);
+    if (x >= F.width || y >= F.height) return ITEM_SKIP;
+    const size_t p = (size_t)y * F.width + x;
+    const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
+    const bool background = P4.w == 0.0f && N4.w == 0.0f;
+    if (background) return ITEM_SKIP;  // factor 0: no bit
+    const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
+    if (mode == 4) return ITEM_LIT;  // glass: factor 1
+    const v3 P = ld3(P4);
+    const v3 N = normalize(ld3(N4));
+    const hr_light& lt = F.lights[li];
+    const bool isDir = lt.pos_maxdist[3] < 0.0f;
+    v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
+    const float dist = length(L);
+    L = normalize(L);
+    const float NdotL = clamp01(dot(N, L));
+    if (!(NdotL > 0.0f)) return ITEM_SKIP;  // factor 0
+    if (!(isDir || dist < lt.pos_maxdist[3])) return ITEM_LIT;  // beyond the light's reach: factor 1
+    o.P = P;
+    o.L = L;
+    o.dist = dist;
+    o.isDir = isDir;
+    o.x = x;
+    o.y = y;
+    return ITEM_TRACE;
+}
+
 // Ray source of the hybrid shadow items (MODE and MULTI as above).
 template <int MODE, bool MULTI>
 struct HybridItems {
@@ -551,19 +596,20 @@ struct HybridItems {
 
     __device__ __forceinline__ int setup(uint32_t item, uint32_t& bit, RayP& r, float& tmax, const TNode4*& n4,
                                          const STri*& tr, uint32_t& nc, Cnt& c) const {
+        static_assert(MODE >= 1 && MODE <= 3, "MODE 1 and 2: adaptive stages; MODE 3: listed groups");
         uint32_t G, s;  // global group, sample
-        if (MODE == 0) {
-            G = item >> lg2s;
-            s = item & ((1u << lg2s) - 1u);
-        } else if (MODE == 1) {
+        if (MODE == 1) {
             G = item >> 2;
             s = item & 3u;
+        } else if (MODE == 3) {
+            G = list[item >> lg2s];
+            s = item & ((1u << lg2s) - 1u);
         } else {
             const uint32_t slot = item / size;
             G = list[slot];
             s = n0 + (item - slot * size);
         }
-        bit = MODE == 0 ? item : (G << lg2s) | s;
+        bit = (G << lg2s) | s;
         const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
         const FrameArgs& F = J.F;
         const uint32_t SS = F.samples;
@@ -572,30 +618,17 @@ struct HybridItems {
         const uint32_t per_tile = F.nl * 64u;
         const uint32_t t = g / per_tile, rem = g - t * per_tile;
         const uint32_t li = rem >> 6, q = rem & 63u;
-        uint32_t x, y;
-        tile_pixel(J, t, q, x, y);
-        if (x >= F.width || y >= F.height) return ITEM_SKIP;
-        const bool hard = MODE != 0 && hard_light(F, li);
+        const bool hard = (MODE == 1 || MODE == 2) && hard_light(F, li);
         if (hard && s > 0u) return ITEM_SKIP;  // a hard light takes one probe
-        const size_t p = (size_t)y * F.width + x;
-        const float4 P4 = J.G.pos[p], N4 = J.G.nrm[p];
-        const bool background = P4.w == 0.0f && N4.w == 0.0f;
-        if (background) return ITEM_SKIP;  // factor 0: no bit
-        const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
-        if (mode == 4) return ITEM_LIT;  // glass: factor 1
-        const v3 P = ld3(P4);
-        const v3 N = normalize(ld3(N4));
-        const hr_light& lt = F.lights[li];
-        const bool isDir = lt.pos_maxdist[3] < 0.0f;
-        v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
-        const float dist = length(L);
-        L = normalize(L);
-        const float NdotL = clamp01(dot(N, L));
-        if (!(NdotL > 0.0f)) return ITEM_SKIP;  // factor 0
-        if (!(isDir || dist < lt.pos_maxdist[3])) return ITEM_LIT;  // beyond the light's reach: factor 1
-        const uint32_t seed = tea(y * F.width + x, F.frame);
+        GroupRay gr;
+        const int kind = group_kind(J, t, li, q, gr);
+        if (kind != ITEM_TRACE) return kind;
+        const v3 P = gr.P, L = gr.L;
+        const float dist = gr.dist;
+        const bool isDir = gr.isDir;
+        const uint32_t seed = tea(gr.y * F.width + gr.x, F.frame);
         v3 dir;
-        if (MODE == 0) {
+        if (MODE == 3) {
             Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
             dir = light_sample_dir(F, li, P, L, gs, s, SS);
         } else {  // adaptive: sample s of the light's nested sequence
@@ -625,6 +658,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
     if (MODE == 2) nitems = *list_count * size;
+    if (MODE == 3) nitems = *list_count << lg2s;
     HybridItems<MODE, MULTI> src{jobs, njobs, lg2s, list, n0, size, jobs[0].S.nodes4, jobs[0].S.tris,
                                  jobs[0].S.node_count};
     item_loop<OFF32>(src, nitems, next_item, bits, stk + threadIdx.x, c);
@@ -665,6 +699,143 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
     const uint32_t n = window_bits(bits, (uint64_t)G << lg2s, lg2s);
     put_factor(J.shadow, li * ((size_t)F.width * F.height) + (size_t)y * F.width + x, (float)n / (float)F.samples,
                F.unorm8);
+}
+
+// Non-adaptive passes: every (pixel, light) group decided once before the
+// samples are traced. A group lit without rays gets the first F.samples bits
+// of its window (the bits its samples would set one by one), a group without
+// rays and factor 0 gets nothing, and the groups to trace are listed in group
+// order (so the trace's work bands keep neighbouring tiles together): each
+// chunk of 64 groups records the mask of its groups to trace, the chunks'
+// counts are scanned, and the groups are scattered to their places. The
+// trace then takes only the listed groups' samples (MODE 3).
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict__ jobs, uint32_t njobs,
+                                                     uint32_t ngroups, uint32_t lg2s, uint32_t* bits,
+                                                     unsigned long long* masks, uint32_t* counts) {
+    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
+    int kind = ITEM_SKIP;
+    if (G < ngroups) {
+        const ShadowJob& J = jobs[MULTI ? job_of(jobs, njobs, G) : 0u];
+        const FrameArgs& F = J.F;
+        const uint32_t g = G - J.group_base;
+        const uint32_t per_tile = F.nl * 64u;
+        const uint32_t t = g / per_tile, rem = g - t * per_tile;
+        GroupRay gr;
+        kind = group_kind(J, t, rem >> 6, rem & 63u, gr);
+        if (kind == ITEM_LIT) {
+            uint64_t b = (uint64_t)G << lg2s;
+            for (uint32_t n = F.samples; n > 0u;) {
+                const uint32_t off = (uint32_t)(b & 31u), take = min(n, 32u - off);
+                const uint32_t m = (take == 32u ? 0xFFFFFFFFu : ((1u << take) - 1u)) << off;
+                atomicOr(bits + (b >> 5), m);
+                b += take;
+                n -= take;
+            }
+        }
+    }
+    const unsigned long long want = __ballot(kind == ITEM_TRACE);
+    if ((threadIdx.x & 63u) == 0u && G < ngroups) {
+        masks[G >> 6] = want;
+        counts[G >> 6] = (uint32_t)__popcll(want);
+    }
+}
+
+// Exclusive prefix sums of the chunk counts in place, in three launches:
+// k_tile_totals sums each tile of SCAN_TILE counts, k_scan_counts scans the
+// tile totals (one workgroup; *total gets the sum of all), k_tile_offsets
+// scans each tile through LDS and adds its tile's offset.
+constexpr uint32_t SCAN_TILE = 1024, SCAN_TILE_THREADS = 256;
+__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_totals(const uint32_t* counts, uint32_t n,
+                                                                    uint32_t* totals) {
+    __shared__ uint32_t red[SCAN_TILE_THREADS];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_TILE / SCAN_TILE_THREADS; ++k) {
+        const uint32_t i = base + k * SCAN_TILE_THREADS + threadIdx.x;
+        if (i < n) s += counts[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = SCAN_TILE_THREADS / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_offsets(uint32_t* counts, uint32_t n,
+                                                                     const uint32_t* tile_offsets) {
+    constexpr uint32_t PER = SCAN_TILE / SCAN_TILE_THREADS;
+    __shared__ uint32_t v[SCAN_TILE];
+    __shared__ uint32_t part[SCAN_TILE_THREADS];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
+        v[j] = i < n ? counts[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) s += v[threadIdx.x * PER + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_TILE_THREADS; d <<= 1) {
+        const uint32_t x = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = tile_offsets[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = threadIdx.x * PER + k;
+        const uint32_t c = v[j];
+        v[j] = run;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
+        if (i < n) counts[i] = v[j];
+    }
+}
+
+// exclusive prefix sums of counts[0, n) in place, the total to *total: one
+// workgroup, each thread a contiguous run (for the tile totals)
+constexpr uint32_t SCAN_BLOCK = 1024;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_counts(uint32_t* counts, uint32_t n, uint32_t* total) {
+    __shared__ uint32_t part[SCAN_BLOCK];
+    const uint32_t per = (n + SCAN_BLOCK - 1u) / SCAN_BLOCK;
+    const uint32_t b = threadIdx.x * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_BLOCK; d <<= 1) {
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;  // exclusive
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == SCAN_BLOCK - 1u) *total = part[SCAN_BLOCK - 1u];
+}
+
+__global__ __launch_bounds__(256) void k_group_scatter(uint32_t ngroups, const unsigned long long* masks,
+                                                       const uint32_t* offsets, uint32_t* list) {
+    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= ngroups) return;
+    const unsigned long long m = masks[G >> 6];
+    const uint32_t lane = G & 63u;
+    if ((m >> lane) & 1ull) list[offsets[G >> 6] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = G;
 }
 
 // unoccluded samples among the first N (<= 2^lg2s <= 64) of the window at bit b0
@@ -749,7 +920,11 @@ static uint32_t host_adaptive_next(uint32_t N, uint32_t S) {
 
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
-    if (!adaptive) return w;
+    // bits, the list of groups to trace and its count, per 64-group chunk an
+    // 8-byte mask (after up to 7 alignment bytes) and a count, and the chunk
+    // counts' tile totals
+    const size_t nchunks = (ngroups + 63u) / 64u;
+    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + 7u + nchunks * 12u + ((nchunks + 1023u) / 1024u) * 4u;
     // bits, two lists of groups, their two counts on lines of their own
     return w + 2u * (size_t)ngroups * 4u + 2u * 256u;
 }
@@ -768,8 +943,28 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
     if (!adaptive) {
-        launch_spl_mode<0>(multi, off32, nb, st, jobs_dev, njobs, (uint32_t)items, lg2s, next_item, bits, nullptr,
-                           nullptr, 0u, 0u, counters);
+        // scratch: bits, the list of groups to trace, its count, and per
+        // chunk of 64 groups the mask of groups to trace (8-byte aligned) and
+        // the count, then offset, of its groups in the list
+        const uint32_t nchunks = (ngroups + 63u) / 64u;
+        uint32_t* list = bits + words;
+        uint32_t* count = list + ngroups;
+        unsigned long long* masks =
+            reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(count + 1) + 7u & ~(uintptr_t)7u);
+        uint32_t* chunk_counts = reinterpret_cast<uint32_t*>(masks + nchunks);
+        const uint32_t gblocks = (ngroups + 255u) / 256u;
+        if (multi)
+            k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, masks, chunk_counts);
+        else
+            k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, masks, chunk_counts);
+        const uint32_t ntiles = (nchunks + SCAN_TILE - 1u) / SCAN_TILE;
+        uint32_t* tile_totals = chunk_counts + nchunks;
+        k_tile_totals<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
+        k_scan_counts<<<1, SCAN_BLOCK, 0, st>>>(tile_totals, ntiles, count);
+        k_tile_offsets<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
+        k_group_scatter<<<gblocks, 256, 0, st>>>(ngroups, masks, chunk_counts, list);
+        launch_spl_mode<3>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, 0u, 0u,
+                           counters);
         k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
