@@ -173,11 +173,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t s_base) {
 // input halo, all Cin channels, into LDS (zero outside the image; pixel
 // stride Cin + 8 elements against bank conflicts), so the 9 taps read their
 // fragments from LDS instead of re-reading global memory. Same K walk,
-// fragments and MFMA order per K step as k_conv.
+// fragments and MFMA order per K step as k_conv. TWO: the halo's channels from
+// cin1 on come from in2 at half resolution (ConvArgs).
 constexpr int LT_H = 8, LT_W = 32, HALO_H = LT_H + 2, HALO_W = LT_W + 2;
 constexpr int HALO_BATCH = 8;
 
-template <int NS>
+template <int NS, bool TWO>
 __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t s_base) {
     extern __shared__ __align__(16) __bf16 tile[];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -205,9 +206,17 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
             if (i < total) {
                 const uint32_t hp = i / cpp, ch = i - hp * cpp;
                 const int y = y0 + (int)(hp / HALO_W) - 1, x = x0 + (int)(hp % HALO_W) - 1;
-                if (y >= 0 && y < H && x >= 0 && x < W)
-                    v[j] = *reinterpret_cast<const bf16x8*>(
-                        a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C + a.in_off + ch * 8u);
+                if (y >= 0 && y < H && x >= 0 && x < W) {
+                    if (!TWO || ch < (a.cin1 >> 3))
+                        v[j] = *reinterpret_cast<const bf16x8*>(
+                            a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C + a.in_off +
+                            ch * 8u);
+                    else
+                        v[j] = *reinterpret_cast<const bf16x8*>(
+                            a.in2 + ((size_t)(n * (a.H >> 1) + ((uint32_t)y >> 1)) * (a.W >> 1) + ((uint32_t)x >> 1)) *
+                                        a.in2_C +
+                            (ch * 8u - a.cin1));
+                }
                 dst[j] = hp * ps + ch * 8u;
             }
         }
@@ -699,7 +708,8 @@ template <int NS>
 static void conv_launch(const ConvArgs& a, uint32_t ny, uint32_t s_base, bool lds_path, size_t lds, hipStream_t st) {
     if (lds_path) {
         const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
-        dn::k_conv_lds<NS><<<dim3(a.N * tx * ty, ny), 256, lds, st>>>(a, tx, ty, s_base);
+        if (a.in2) dn::k_conv_lds<NS, true><<<dim3(a.N * tx * ty, ny), 256, lds, st>>>(a, tx, ty, s_base);
+        else dn::k_conv_lds<NS, false><<<dim3(a.N * tx * ty, ny), 256, lds, st>>>(a, tx, ty, s_base);
     } else {
         const size_t P = (size_t)a.N * a.H * a.W;
         dn::k_conv<NS><<<dim3(nblk(P, 4 * 16 * dn::CONV_MT), ny), 256, 0, st>>>(a, s_base);
@@ -711,6 +721,8 @@ hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
     // gfx950 has 160 KiB per CU) and 16-byte chunks stay aligned
     const size_t lds = (size_t)dn::HALO_H * dn::HALO_W * (a.Cin + 8u) * sizeof(__bf16);
     const bool lds_path = lds <= 98304u && (a.Cin & 7u) == 0 && (a.in_C & 7u) == 0 && (a.in_off & 7u) == 0;
+    if (a.in2 && (!lds_path || (a.cin1 & 7u) || (a.in2_C & 7u) || (a.H & 1u) || (a.W & 1u)))
+        return hipErrorInvalidValue;  // two inputs: LDS path, 8-channel chunks, even sizes
     const uint32_t ns_total = a.Cout_pad >> 4;
     const uint32_t full = ns_total / dn::CONV_NT, rem = ns_total % dn::CONV_NT;
     static_assert(dn::CONV_NT == 4, "remainder launches below cover 1..3 tiles");

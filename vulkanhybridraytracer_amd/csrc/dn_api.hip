@@ -271,10 +271,30 @@ KpcnArgs kpcn_args(const Arena& A, const hr_dn_image& im, uint32_t loss) {
     return k;
 }
 
+// The upsampling that fills the second part of concatenation buffer `buf`
+// (false: `buf` is not one).
+bool concat_of(int buf, Op& up) {
+    for (const Op& o : OPS)
+        if (o.kind == OP_UP && o.dst == buf) {
+            up = o;
+            return true;
+        }
+    return false;
+}
+
+// Inference (no gradients) writes no concatenation: a concatenation buffer
+// holds only its encoder output, compact (channel stride = the encoder's
+// channels, dst_off of the upsampling), the pooling reads it there, and the
+// decoder layer that reads the concatenation takes its second part straight
+// from the upsampling's half-resolution source (two-input convolution). The
+// values are those of the written-out concatenation.
 hr_status forward(hr_denoiser* dn, const Arena& A) {
     hr_context* ctx = dn->ctx;
     hipStream_t st = ctx->stream;
+    const bool fused = !A.grads;
     for (const Op& op : OPS) {
+        Op up{};
+        if (fused && op.kind == OP_UP) continue;
         if (op.kind == OP_CONV) {
             const LayerDesc& d = LAYERS[op.layer];
             const LayerInt& li = dn->L[op.layer];
@@ -301,11 +321,20 @@ hr_status forward(hr_denoiser* dn, const Arena& A) {
             a.Cout = d.cout;
             a.Cout_pad = li.cout_pad;
             a.relu = op.layer != L_HEAD;
+            if (fused && op.dst != B_LOGITS && concat_of(op.dst, up)) a.out_C = up.dst_off;
+            if (fused && concat_of(op.src, up)) {
+                if (op.src_off != 0u || up.dst_off + up.C != li.cin_pad) return fail(ctx, HR_E_INVALID, "concat layout");
+                a.in_C = up.dst_off;
+                a.in2 = A.act[up.src];
+                a.in2_C = BUFS[up.src].C;
+                a.cin1 = up.dst_off;
+            }
             HR_HIP(ctx, dn_conv(a, st));
         } else if (op.kind == OP_POOL) {
             uint32_t lvl = BUFS[op.dst].level;
-            HR_HIP(ctx, dn_pool(A.act[op.src], BUFS[op.src].C, op.src_off, A.act[op.dst], op.C, A.N, A.Hp >> lvl,
-                                A.Wp >> lvl, st));
+            const uint32_t in_C = fused && concat_of(op.src, up) ? up.dst_off : BUFS[op.src].C;
+            HR_HIP(ctx, dn_pool(A.act[op.src], in_C, op.src_off, A.act[op.dst], op.C, A.N, A.Hp >> lvl, A.Wp >> lvl,
+                                st));
         } else {
             uint32_t lvl = BUFS[op.dst].level;
             HR_HIP(ctx, dn_up(A.act[op.src], A.act[op.dst], BUFS[op.dst].C, op.dst_off, op.C, A.N, A.Hp >> lvl,

@@ -289,6 +289,12 @@ struct ConvArgs {
     uint32_t in_C, in_off, Cin, K, Kpad;  // K = 9 * Cin, Kpad = K rounded up to 32
     uint32_t out_C, out_off, Cout, Cout_pad;
     uint32_t relu;
+    // two inputs (in2 not null; the LDS path only): channels [0, cin1) from
+    // in (at in_off, stride in_C), channels [cin1, Cin) from in2 (stride
+    // in2_C) at half resolution, pixel (x >> 1, y >> 1): a concatenation with
+    // a nearest 2x upsampling, without either written out
+    const __bf16* in2;
+    uint32_t in2_C, cin1;
 };
 struct WgradArgs {
     const __bf16* dout;    // gradient of the layer output (after the ReLU mask)
