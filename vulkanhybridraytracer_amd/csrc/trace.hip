@@ -599,7 +599,7 @@ struct HybridItems {
         static_assert(MODE >= 1 && MODE <= 3, "MODE 1 and 2: adaptive stages; MODE 3: listed groups");
         uint32_t G, s;  // global group, sample
         if (MODE == 1) {
-            G = item >> 2;
+            G = list[item >> 2];
             s = item & 3u;
         } else if (MODE == 3) {
             G = list[item >> lg2s];
@@ -657,6 +657,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
     static_assert(!(MULTI && OFF32), "frames of a batch have their own node arrays");
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
+    if (MODE == 1) nitems = *list_count << 2;
     if (MODE == 2) nitems = *list_count * size;
     if (MODE == 3) nitems = *list_count << lg2s;
     HybridItems<MODE, MULTI> src{jobs, njobs, lg2s, list, n0, size, jobs[0].S.nodes4, jobs[0].S.tris,
@@ -709,10 +710,13 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
 // chunk of 64 groups records the mask of its groups to trace, the chunks'
 // counts are scanned, and the groups are scattered to their places. The
 // trace then takes only the listed groups' samples (MODE 3).
+// probes: 0 without adaptive sampling (a lit group's F.samples bits), else
+// the probe count of the adaptive first stage (a lit group's probe bits: one
+// for a hard light, else min(F.samples, probes)).
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict__ jobs, uint32_t njobs,
-                                                     uint32_t ngroups, uint32_t lg2s, uint32_t* bits,
-                                                     unsigned long long* masks, uint32_t* counts) {
+                                                     uint32_t ngroups, uint32_t lg2s, uint32_t probes,
+                                                     uint32_t* bits, unsigned long long* masks, uint32_t* counts) {
     const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = ITEM_SKIP;
     if (G < ngroups) {
@@ -721,11 +725,13 @@ __global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict
         const uint32_t g = G - J.group_base;
         const uint32_t per_tile = F.nl * 64u;
         const uint32_t t = g / per_tile, rem = g - t * per_tile;
+        const uint32_t li = rem >> 6;
         GroupRay gr;
-        kind = group_kind(J, t, rem >> 6, rem & 63u, gr);
+        kind = group_kind(J, t, li, rem & 63u, gr);
         if (kind == ITEM_LIT) {
             uint64_t b = (uint64_t)G << lg2s;
-            for (uint32_t n = F.samples; n > 0u;) {
+            const uint32_t lit = probes == 0u ? F.samples : (hard_light(F, li) ? 1u : min(F.samples, probes));
+            for (uint32_t n = lit; n > 0u;) {
                 const uint32_t off = (uint32_t)(b & 31u), take = min(n, 32u - off);
                 const uint32_t m = (take == 32u ? 0xFFFFFFFFu : ((1u << take) - 1u)) << off;
                 atomicOr(bits + (b >> 5), m);
@@ -918,15 +924,44 @@ static uint32_t host_adaptive_next(uint32_t N, uint32_t S) {
     return n < S ? n : S;
 }
 
+// scratch of list_groups after its list and count: up to 7 alignment bytes,
+// per 64-group chunk an 8-byte mask and a count, the chunk counts' tile totals
+static size_t list_tail_bytes(uint32_t ngroups) {
+    const size_t nchunks = (ngroups + 63u) / 64u;
+    return 7u + nchunks * 12u + ((nchunks + SCAN_TILE - 1u) / SCAN_TILE) * 4u;
+}
+
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
-    // bits, the list of groups to trace and its count, per 64-group chunk an
-    // 8-byte mask (after up to 7 alignment bytes) and a count, and the chunk
-    // counts' tile totals
-    const size_t nchunks = (ngroups + 63u) / 64u;
-    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + 7u + nchunks * 12u + ((nchunks + 1023u) / 1024u) * 4u;
-    // bits, two lists of groups, their two counts on lines of their own
-    return w + 2u * (size_t)ngroups * 4u + 2u * 256u;
+    // bits, the list of groups to trace and its count, the tail of list_groups
+    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + list_tail_bytes(ngroups);
+    // bits, two lists of groups, their two counts on lines of their own, the
+    // tail of list_groups
+    return w + 2u * (size_t)ngroups * 4u + 2u * 256u + list_tail_bytes(ngroups);
+}
+
+// The groups that need rays, listed in group order into list / *count (the
+// others settled in bits, see k_group_kinds); tail: scratch for the chunk
+// masks, counts and tile totals (list_tail_bytes).
+static void list_groups(bool multi, const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
+                 uint32_t probes, uint32_t* bits, uint32_t* list, uint32_t* count, void* tail, hipStream_t st) {
+    const uint32_t nchunks = (ngroups + 63u) / 64u;
+    unsigned long long* masks =
+        reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(tail) + 7u & ~(uintptr_t)7u);
+    uint32_t* chunk_counts = reinterpret_cast<uint32_t*>(masks + nchunks);
+    uint32_t* tile_totals = chunk_counts + nchunks;
+    const uint32_t gblocks = (ngroups + 255u) / 256u;
+    if (multi)
+        k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, masks,
+                                                     chunk_counts);
+    else
+        k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, masks,
+                                                      chunk_counts);
+    const uint32_t ntiles = (nchunks + SCAN_TILE - 1u) / SCAN_TILE;
+    k_tile_totals<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
+    k_scan_counts<<<1, SCAN_BLOCK, 0, st>>>(tile_totals, ntiles, count);
+    k_tile_offsets<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
+    k_group_scatter<<<gblocks, 256, 0, st>>>(ngroups, masks, chunk_counts, list);
 }
 
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
@@ -943,41 +978,28 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
     if (!adaptive) {
-        // scratch: bits, the list of groups to trace, its count, and per
-        // chunk of 64 groups the mask of groups to trace (8-byte aligned) and
-        // the count, then offset, of its groups in the list
-        const uint32_t nchunks = (ngroups + 63u) / 64u;
+        // scratch: bits, the list of groups to trace, its count, the tail
         uint32_t* list = bits + words;
         uint32_t* count = list + ngroups;
-        unsigned long long* masks =
-            reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(count + 1) + 7u & ~(uintptr_t)7u);
-        uint32_t* chunk_counts = reinterpret_cast<uint32_t*>(masks + nchunks);
-        const uint32_t gblocks = (ngroups + 255u) / 256u;
-        if (multi)
-            k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, masks, chunk_counts);
-        else
-            k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, masks, chunk_counts);
-        const uint32_t ntiles = (nchunks + SCAN_TILE - 1u) / SCAN_TILE;
-        uint32_t* tile_totals = chunk_counts + nchunks;
-        k_tile_totals<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
-        k_scan_counts<<<1, SCAN_BLOCK, 0, st>>>(tile_totals, ntiles, count);
-        k_tile_offsets<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
-        k_group_scatter<<<gblocks, 256, 0, st>>>(ngroups, masks, chunk_counts, list);
+        list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 0u, bits, list, count, count + 1, st);
         launch_spl_mode<3>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, 0u, 0u,
                            counters);
         k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
     // scratch: bits (2^lg2s per group, every stage adds its own), lists A and
-    // B, their counts 64 words apart
+    // B, their counts 64 words apart, the tail of list_groups. The probe
+    // stage traces the groups listed in B (free until the first refinement
+    // stage lists into it).
     uint32_t* lists[2] = {bits + words, bits + words + ngroups};
     uint32_t* counts[2] = {bits + words + 2u * (size_t)ngroups, bits + words + 2u * (size_t)ngroups + 64u};
     const uint32_t S = 1u << lg2s;
     const uint32_t gb = (ngroups + 255u) / 256u;
     k_fill_bits<<<1, 64, 0, st>>>(counts[0], 1u);
+    list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 4u, bits, lists[1], counts[1], counts[1] + 64u, st);
     const uint32_t pitems = ngroups << 2;
-    launch_spl_mode<1>(multi, off32, min(resident_blocks, (pitems + BLOCK - 1) / BLOCK), st, jobs_dev, njobs, pitems,
-                       lg2s, next_item, bits, nullptr, nullptr, 0u, 4u, counters);
+    launch_spl_mode<1>(multi, off32, min(resident_blocks, (pitems + BLOCK - 1) / BLOCK), st, jobs_dev, njobs, 0u,
+                       lg2s, next_item, bits, lists[1], counts[1], 0u, 4u, counters);
     k_adaptive_decide<true><<<gb, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, nullptr, nullptr, 4u,
                                                 lists[0], counts[0]);
     int cur = 0;
