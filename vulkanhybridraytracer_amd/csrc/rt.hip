@@ -268,7 +268,9 @@ struct RtWork {
     float4 *hpos, *hnrm;
     uint32_t* hpix;
     uint32_t* bits;
-    uint32_t* cnt;  // [0], [1]: live list lengths, [2]: hits of the current depth
+    uint32_t* cnt;  // [0], [1]: live list lengths, [2]: hits of the current depth, [3]: listed groups
+    uint32_t* glist;  // the (light, hit) groups of the current depth that need shadow rays, in group order
+    void* gtail;      // scratch of their listing
 };
 
 __device__ __forceinline__ uint32_t rt_tiles(const FrameArgs& F) {
@@ -283,6 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_rt_begin(FrameArgs F, RtWork w, uint3
         w.cnt[0] = T;
         w.cnt[1] = 0u;
         w.cnt[2] = 0u;
+        w.cnt[3] = 0u;
     }
     if (i >= T) return;
     const uint32_t t = i >> 6, q = i & 63u, tx = (F.width + 7u) / 8u;
@@ -301,10 +304,12 @@ __global__ __launch_bounds__(BLOCK) void k_rt_begin(FrameArgs F, RtWork w, uint3
     if (nS == 0u) w.fin[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// before depth `depth`: no hits yet, the next live list empty, work bands reset
+// before depth `depth`: no hits or listed groups yet, the next live list
+// empty, work bands reset
 __global__ void k_rt_reset(RtWork w, uint32_t depth, uint32_t* next_item) {
     if (threadIdx.x == 0) {
         w.cnt[2] = 0u;
+        w.cnt[3] = 0u;
         w.cnt[(depth + 1u) & 1u] = 0u;
     }
     if (threadIdx.x < SHADOW_QUEUES) next_item[threadIdx.x * BAND_STRIDE] = 0u;
@@ -359,6 +364,27 @@ __global__ __launch_bounds__(BLOCK) void k_rt_closest(DevScene S, FrameArgs F, R
     flush_counters(c, counters, false);
 }
 
+// The (light, hit) groups G = light * hits + slot of the current depth whose
+// shadow rays are traced (the light in front of the surface and within reach;
+// the others' samples set no bit): per chunk of 64 groups the ballot of them
+// and its count, for every chunk of the frame's largest group count.
+__global__ __launch_bounds__(256) void k_rt_group_kinds(FrameArgs F, RtWork w, uint32_t ngroups_max,
+                                                        unsigned long long* masks, uint32_t* counts) {
+    const uint32_t G = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t nh = w.cnt[2];
+    bool trace = false;
+    if (G < F.nl * nh) {
+        const uint32_t li = G / nh, slot = G - li * nh;
+        const RtLight lg = rt_light(F, li, ld3(w.hpos[slot]), ld3(w.hnrm[slot]));
+        trace = lg.NdotL > 0.0f && lg.dist < lg.maxd;
+    }
+    const unsigned long long m = __ballot(trace);
+    if ((threadIdx.x & 63u) == 0u && G < ngroups_max) {
+        masks[G >> 6] = m;
+        counts[G >> 6] = (uint32_t)__popcll(m);
+    }
+}
+
 // clear the shadow bits of the current depth's items
 __global__ __launch_bounds__(256) void k_rt_clear_bits(RtWork w, uint32_t nl, uint32_t lg2s) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -373,12 +399,14 @@ __device__ __forceinline__ Pcg stream_at_sample(uint32_t seed, uint32_t index, u
     return g;
 }
 
-// ray source of the hit shadow items
+// ray source of the hit shadow items: the samples of the listed groups (item
+// = list slot * 2^lg2s + sample; its bit (group << lg2s) | sample)
 struct RtItems {
     const FrameArgs& F;
     const float4* __restrict__ hpos;
     const float4* __restrict__ hnrm;
     const uint32_t* __restrict__ hpix;
+    const uint32_t* __restrict__ glist;
     uint32_t nh, lg2s, nS, depth;
     const TNode4* nodes4;
     const STri* tris;
@@ -386,8 +414,8 @@ struct RtItems {
 
     __device__ __forceinline__ int setup(uint32_t item, uint32_t& bit, RayP& r, float& tmax, const TNode4*&,
                                          const STri*&, uint32_t&, Cnt& c) const {
-        bit = item;
-        const uint32_t G = item >> lg2s, a = item & ((1u << lg2s) - 1u);
+        const uint32_t G = glist[item >> lg2s], a = item & ((1u << lg2s) - 1u);
+        bit = (G << lg2s) | a;
         const uint32_t SS = F.samples;
         if (a >= SS) return ITEM_SKIP;
         const uint32_t li = G / nh, slot = G - li * nh;
@@ -410,8 +438,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
     const uint32_t nh = w.cnt[2];
-    RtItems src{F, w.hpos, w.hnrm, w.hpix, nh, lg2s, nS, depth, S.nodes4, S.tris, S.node_count};
-    item_loop<OFF32>(src, (F.nl * nh) << lg2s, next_item, w.bits, stk + threadIdx.x, c);
+    RtItems src{F, w.hpos, w.hnrm, w.hpix, w.glist, nh, lg2s, nS, depth, S.nodes4, S.tris, S.node_count};
+    item_loop<OFF32>(src, w.cnt[3] << lg2s, next_item, w.bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
 }
 
@@ -496,6 +524,9 @@ size_t rt_layout(const FrameArgs& f, char* base, RtWork& w) {
     w.list[1] = reinterpret_cast<uint32_t*>(take(T * 4));
     w.bits = reinterpret_cast<uint32_t*>(take((((uint64_t)f.nl * P << lg2s) + 31u) / 32u * 4u));
     w.cnt = reinterpret_cast<uint32_t*>(take(64));
+    const uint32_t groups = f.nl * (uint32_t)P;
+    w.glist = reinterpret_cast<uint32_t*>(take((size_t)groups * 4u));
+    w.gtail = take(group_list_tail_bytes(groups));
     return off;
 }
 
@@ -521,12 +552,18 @@ hipError_t launch_rt_wavefront(const DevScene& s, const FrameArgs& f, float4* ac
     const uint64_t max_words = ((((uint64_t)f.nl * P) << lg2s) + 31u) >> 5;
     const uint32_t nbW = (uint32_t)((max_words + 255u) / 256u);
     const uint32_t spp = max(f.spp, 1u);
+    const uint32_t gmax = f.nl * P;  // (light, hit) groups of a depth at most
+    const GroupListTail gt = group_list_tail(w.gtail, gmax);
     for (uint32_t nS = 0; nS < spp; ++nS) {
         k_rt_begin<<<nbT, BLOCK, 0, st>>>(f, w, nS);
         for (uint32_t depth = 0; depth < f.max_depth; ++depth) {
             k_rt_reset<<<1, 64, 0, st>>>(w, depth, next_item);
             k_rt_closest<<<nbT, BLOCK, 0, st>>>(s, f, w, depth, counters);
             if (nbW) k_rt_clear_bits<<<nbW, 256, 0, st>>>(w, f.nl, lg2s);
+            if (gmax) {
+                k_rt_group_kinds<<<(gmax + 255u) / 256u, 256, 0, st>>>(f, w, gmax, gt.masks, gt.counts);
+                scan_and_scatter(gmax, gt, w.cnt + 3, w.glist, st);
+            }
             if ((uint64_t)s.wide_count * sizeof(TNode4) <= 0xFFFFFFFFull)
                 k_rt_shadows<true><<<resident_blocks, BLOCK, 0, st>>>(s, f, w, lg2s, nS, depth, next_item, counters);
             else

@@ -747,103 +747,6 @@ __global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict
     }
 }
 
-// Exclusive prefix sums of the chunk counts in place, in three launches:
-// k_tile_totals sums each tile of SCAN_TILE counts, k_scan_counts scans the
-// tile totals (one workgroup; *total gets the sum of all), k_tile_offsets
-// scans each tile through LDS and adds its tile's offset.
-constexpr uint32_t SCAN_TILE = 1024, SCAN_TILE_THREADS = 256;
-__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_totals(const uint32_t* counts, uint32_t n,
-                                                                    uint32_t* totals) {
-    __shared__ uint32_t red[SCAN_TILE_THREADS];
-    const uint32_t base = blockIdx.x * SCAN_TILE;
-    uint32_t s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < SCAN_TILE / SCAN_TILE_THREADS; ++k) {
-        const uint32_t i = base + k * SCAN_TILE_THREADS + threadIdx.x;
-        if (i < n) s += counts[i];
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t d = SCAN_TILE_THREADS / 2; d > 0; d >>= 1) {
-        if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) totals[blockIdx.x] = red[0];
-}
-__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_offsets(uint32_t* counts, uint32_t n,
-                                                                     const uint32_t* tile_offsets) {
-    constexpr uint32_t PER = SCAN_TILE / SCAN_TILE_THREADS;
-    __shared__ uint32_t v[SCAN_TILE];
-    __shared__ uint32_t part[SCAN_TILE_THREADS];
-    const uint32_t base = blockIdx.x * SCAN_TILE;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
-        v[j] = i < n ? counts[i] : 0u;
-    }
-    __syncthreads();
-    uint32_t s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) s += v[threadIdx.x * PER + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t d = 1; d < SCAN_TILE_THREADS; d <<= 1) {
-        const uint32_t x = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += x;
-        __syncthreads();
-    }
-    uint32_t run = tile_offsets[blockIdx.x] + part[threadIdx.x] - s;
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t j = threadIdx.x * PER + k;
-        const uint32_t c = v[j];
-        v[j] = run;
-        run += c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
-        if (i < n) counts[i] = v[j];
-    }
-}
-
-// exclusive prefix sums of counts[0, n) in place, the total to *total: one
-// workgroup, each thread a contiguous run (for the tile totals)
-constexpr uint32_t SCAN_BLOCK = 1024;
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_counts(uint32_t* counts, uint32_t n, uint32_t* total) {
-    __shared__ uint32_t part[SCAN_BLOCK];
-    const uint32_t per = (n + SCAN_BLOCK - 1u) / SCAN_BLOCK;
-    const uint32_t b = threadIdx.x * per, e = min(n, b + per);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += counts[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t d = 1; d < SCAN_BLOCK; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;  // exclusive
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t c = counts[i];
-        counts[i] = run;
-        run += c;
-    }
-    if (threadIdx.x == SCAN_BLOCK - 1u) *total = part[SCAN_BLOCK - 1u];
-}
-
-__global__ __launch_bounds__(256) void k_group_scatter(uint32_t ngroups, const unsigned long long* masks,
-                                                       const uint32_t* offsets, uint32_t* list) {
-    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
-    if (G >= ngroups) return;
-    const unsigned long long m = masks[G >> 6];
-    const uint32_t lane = G & 63u;
-    if ((m >> lane) & 1ull) list[offsets[G >> 6] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = G;
-}
-
 // unoccluded samples among the first N (<= 2^lg2s <= 64) of the window at bit b0
 __device__ __forceinline__ uint32_t prefix_bits(const uint32_t* __restrict__ bits, uint64_t b0, uint32_t lg2s,
                                                 uint32_t N) {
@@ -924,20 +827,13 @@ static uint32_t host_adaptive_next(uint32_t N, uint32_t S) {
     return n < S ? n : S;
 }
 
-// scratch of list_groups after its list and count: up to 7 alignment bytes,
-// per 64-group chunk an 8-byte mask and a count, the chunk counts' tile totals
-static size_t list_tail_bytes(uint32_t ngroups) {
-    const size_t nchunks = (ngroups + 63u) / 64u;
-    return 7u + nchunks * 12u + ((nchunks + SCAN_TILE - 1u) / SCAN_TILE) * 4u;
-}
-
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
     // bits, the list of groups to trace and its count, the tail of list_groups
-    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + list_tail_bytes(ngroups);
+    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + group_list_tail_bytes(ngroups);
     // bits, two lists of groups, their two counts on lines of their own, the
     // tail of list_groups
-    return w + 2u * (size_t)ngroups * 4u + 2u * 256u + list_tail_bytes(ngroups);
+    return w + 2u * (size_t)ngroups * 4u + 2u * 256u + group_list_tail_bytes(ngroups);
 }
 
 // The groups that need rays, listed in group order into list / *count (the
@@ -945,23 +841,15 @@ size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
 // masks, counts and tile totals (list_tail_bytes).
 static void list_groups(bool multi, const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
                  uint32_t probes, uint32_t* bits, uint32_t* list, uint32_t* count, void* tail, hipStream_t st) {
-    const uint32_t nchunks = (ngroups + 63u) / 64u;
-    unsigned long long* masks =
-        reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(tail) + 7u & ~(uintptr_t)7u);
-    uint32_t* chunk_counts = reinterpret_cast<uint32_t*>(masks + nchunks);
-    uint32_t* tile_totals = chunk_counts + nchunks;
+    const GroupListTail t = group_list_tail(tail, ngroups);
     const uint32_t gblocks = (ngroups + 255u) / 256u;
     if (multi)
-        k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, masks,
-                                                     chunk_counts);
+        k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, t.masks,
+                                                     t.counts);
     else
-        k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, masks,
-                                                      chunk_counts);
-    const uint32_t ntiles = (nchunks + SCAN_TILE - 1u) / SCAN_TILE;
-    k_tile_totals<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
-    k_scan_counts<<<1, SCAN_BLOCK, 0, st>>>(tile_totals, ntiles, count);
-    k_tile_offsets<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(chunk_counts, nchunks, tile_totals);
-    k_group_scatter<<<gblocks, 256, 0, st>>>(ngroups, masks, chunk_counts, list);
+        k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, t.masks,
+                                                      t.counts);
+    scan_and_scatter(ngroups, t, count, list, st);
 }
 
 hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,

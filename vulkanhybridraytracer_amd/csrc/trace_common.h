@@ -1013,5 +1013,139 @@ __device__ __forceinline__ uint32_t window_bits(const uint32_t* __restrict__ bit
     return (uint32_t)__popc(w & ((1u << (1u << lg2s)) - 1u));
 }
 
+// ---- ordered lists of the groups that need rays -------------------------------
+// A pre-pass writes, per chunk of 64 groups, the ballot of its groups that need
+// rays (masks) and its count (counts); scan_and_scatter turns the counts into
+// offsets and lists the groups in group order (so a trace's work bands keep
+// neighbouring groups together), with the list length at *count.
+namespace {
+
+// Exclusive prefix sums of the chunk counts in place, in three launches:
+// k_tile_totals sums each tile of SCAN_TILE counts, k_scan_counts scans the
+// tile totals (one workgroup; *total gets the sum of all), k_tile_offsets
+// scans each tile through LDS and adds its tile's offset.
+constexpr uint32_t SCAN_TILE = 1024, SCAN_TILE_THREADS = 256;
+__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_totals(const uint32_t* counts, uint32_t n,
+                                                                    uint32_t* totals) {
+    __shared__ uint32_t red[SCAN_TILE_THREADS];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_TILE / SCAN_TILE_THREADS; ++k) {
+        const uint32_t i = base + k * SCAN_TILE_THREADS + threadIdx.x;
+        if (i < n) s += counts[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = SCAN_TILE_THREADS / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(SCAN_TILE_THREADS) void k_tile_offsets(uint32_t* counts, uint32_t n,
+                                                                     const uint32_t* tile_offsets) {
+    constexpr uint32_t PER = SCAN_TILE / SCAN_TILE_THREADS;
+    __shared__ uint32_t v[SCAN_TILE];
+    __shared__ uint32_t part[SCAN_TILE_THREADS];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
+        v[j] = i < n ? counts[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) s += v[threadIdx.x * PER + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_TILE_THREADS; d <<= 1) {
+        const uint32_t x = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t run = tile_offsets[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = threadIdx.x * PER + k;
+        const uint32_t c = v[j];
+        v[j] = run;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t j = k * SCAN_TILE_THREADS + threadIdx.x, i = base + j;
+        if (i < n) counts[i] = v[j];
+    }
+}
+
+// exclusive prefix sums of counts[0, n) in place, the total to *total: one
+// workgroup, each thread a contiguous run (for the tile totals)
+constexpr uint32_t SCAN_BLOCK = 1024;
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_counts(uint32_t* counts, uint32_t n, uint32_t* total) {
+    __shared__ uint32_t part[SCAN_BLOCK];
+    const uint32_t per = (n + SCAN_BLOCK - 1u) / SCAN_BLOCK;
+    const uint32_t b = threadIdx.x * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < SCAN_BLOCK; d <<= 1) {
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;  // exclusive
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == SCAN_BLOCK - 1u) *total = part[SCAN_BLOCK - 1u];
+}
+
+__global__ __launch_bounds__(256) void k_group_scatter(uint32_t ngroups, const unsigned long long* masks,
+                                                       const uint32_t* offsets, uint32_t* list) {
+    const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
+    if (G >= ngroups) return;
+    const unsigned long long m = masks[G >> 6];
+    const uint32_t lane = G & 63u;
+    if ((m >> lane) & 1ull) list[offsets[G >> 6] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = G;
+}
+
+// scratch after a list and its count: up to 7 alignment bytes, per chunk an
+// 8-byte mask and a count, the chunk counts' tile totals
+inline size_t group_list_tail_bytes(uint32_t ngroups) {
+    const size_t nchunks = (ngroups + 63u) / 64u;
+    return 7u + nchunks * 12u + ((nchunks + SCAN_TILE - 1u) / SCAN_TILE) * 4u;
+}
+struct GroupListTail {
+    unsigned long long* masks;
+    uint32_t *counts, *tile_totals;
+    uint32_t nchunks;
+};
+inline GroupListTail group_list_tail(void* tail, uint32_t ngroups) {
+    GroupListTail t;
+    t.nchunks = (ngroups + 63u) / 64u;
+    t.masks = reinterpret_cast<unsigned long long*>(reinterpret_cast<uintptr_t>(tail) + 7u & ~(uintptr_t)7u);
+    t.counts = reinterpret_cast<uint32_t*>(t.masks + t.nchunks);
+    t.tile_totals = t.counts + t.nchunks;
+    return t;
+}
+inline void scan_and_scatter(uint32_t ngroups, const GroupListTail& t, uint32_t* count, uint32_t* list,
+                             hipStream_t st) {
+    const uint32_t ntiles = (t.nchunks + SCAN_TILE - 1u) / SCAN_TILE;
+    k_tile_totals<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(t.counts, t.nchunks, t.tile_totals);
+    k_scan_counts<<<1, SCAN_BLOCK, 0, st>>>(t.tile_totals, ntiles, count);
+    k_tile_offsets<<<ntiles, SCAN_TILE_THREADS, 0, st>>>(t.counts, t.nchunks, t.tile_totals);
+    k_group_scatter<<<(ngroups + 255u) / 256u, 256, 0, st>>>(ngroups, t.masks, t.counts, list);
+}
+
+}  // namespace
+
 }  // namespace tc
 }  // namespace hr
