@@ -553,6 +553,15 @@ struct GroupRay {
     bool isDir;
     uint32_t x, y;
 };
+// the unit vector from P to light lt and its distance (a directional light:
+// its direction)
+__device__ __forceinline__ void light_vector(const hr_light& lt, v3 P, GroupRay& o) {
+    o.isDir = lt.pos_maxdist[3] < 0.0f;
+    v3 L = o.isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
+    o.dist = length(L);
+    o.L = normalize(L);
+    o.P = P;
+}
 __device__ __forceinline__ int group_kind(const ShadowJob& J, uint32_t t, uint32_t li, uint32_t q, GroupRay& o) {
     const FrameArgs& F = J.F;
     uint32_t x, y;
@@ -564,23 +573,21 @@ __device__ __forceinline__ int group_kind(const ShadowJob& J, uint32_t t, uint32
     if (background) return ITEM_SKIP;  // factor 0: no bit
     const int mode = (int)J.S.mats[mat_index(J.S, P4.w)].shading_metal_rough_idx[0];
     if (mode == 4) return ITEM_LIT;  // glass: factor 1
-    const v3 P = ld3(P4);
     const v3 N = normalize(ld3(N4));
     const hr_light& lt = F.lights[li];
-    const bool isDir = lt.pos_maxdist[3] < 0.0f;
-    v3 L = isDir ? ld3p(lt.pos_maxdist) : sub(ld3p(lt.pos_maxdist), P);
-    const float dist = length(L);
-    L = normalize(L);
-    const float NdotL = clamp01(dot(N, L));
+    light_vector(lt, ld3(P4), o);
+    const float NdotL = clamp01(dot(N, o.L));
     if (!(NdotL > 0.0f)) return ITEM_SKIP;  // factor 0
-    if (!(isDir || dist < lt.pos_maxdist[3])) return ITEM_LIT;  // beyond the light's reach: factor 1
-    o.P = P;
-    o.L = L;
-    o.dist = dist;
-    o.isDir = isDir;
+    if (!(o.isDir || o.dist < lt.pos_maxdist[3])) return ITEM_LIT;  // beyond the light's reach: factor 1
     o.x = x;
     o.y = y;
     return ITEM_TRACE;
+}
+// group_kind's ray data of a group it found ITEM_TRACE (the checks skipped)
+__device__ __forceinline__ void group_ray(const ShadowJob& J, uint32_t t, uint32_t li, uint32_t q, GroupRay& o) {
+    const FrameArgs& F = J.F;
+    tile_pixel(J, t, q, o.x, o.y);
+    light_vector(F.lights[li], ld3(J.G.pos[(size_t)o.y * F.width + o.x]), o);
 }
 
 // Ray source of the hybrid shadow items (MODE and MULTI as above).
@@ -589,6 +596,7 @@ struct HybridItems {
     const ShadowJob* __restrict__ jobs;
     uint32_t njobs, lg2s;
     const uint32_t* __restrict__ list;
+    const uint32_t* __restrict__ seeds;  // per group: tea of its pixel and frame
     uint32_t n0, size;  // MODE 2: the stage's first sample and sample count
     const TNode4* nodes4;
     const STri* tris;
@@ -620,13 +628,14 @@ struct HybridItems {
         const uint32_t li = rem >> 6, q = rem & 63u;
         const bool hard = (MODE == 1 || MODE == 2) && hard_light(F, li);
         if (hard && s > 0u) return ITEM_SKIP;  // a hard light takes one probe
+        // a listed group needs rays (k_group_kinds / k_adaptive_decide): its
+        // ray data without the checks, its pixel's seed as k_group_kinds stored it
         GroupRay gr;
-        const int kind = group_kind(J, t, li, q, gr);
-        if (kind != ITEM_TRACE) return kind;
+        group_ray(J, t, li, q, gr);
         const v3 P = gr.P, L = gr.L;
         const float dist = gr.dist;
         const bool isDir = gr.isDir;
-        const uint32_t seed = tea(gr.y * F.width + gr.x, F.frame);
+        const uint32_t seed = seeds[G];
         v3 dir;
         if (MODE == 3) {
             Pcg gs = ray_stream(seed, PURPOSE_SURFACE_SHADOW, li * SS + s);
@@ -652,15 +661,15 @@ struct HybridItems {
 template <int MODE, bool MULTI, bool OFF32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_shadows_spl(
     const ShadowJob* __restrict__ jobs, uint32_t njobs, uint32_t nitems, uint32_t lg2s, uint32_t* next_item,
-    uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, uint32_t n0,
-    uint32_t size, unsigned long long* counters) {
+    uint32_t* bits, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+    const uint32_t* __restrict__ seeds, uint32_t n0, uint32_t size, unsigned long long* counters) {
     static_assert(!(MULTI && OFF32), "frames of a batch have their own node arrays");
     __shared__ uint32_t stk[STACK_LDS * BLOCK];
     Cnt c = {};
     if (MODE == 1) nitems = *list_count << 2;
     if (MODE == 2) nitems = *list_count * size;
     if (MODE == 3) nitems = *list_count << lg2s;
-    HybridItems<MODE, MULTI> src{jobs, njobs, lg2s, list, n0, size, jobs[0].S.nodes4, jobs[0].S.tris,
+    HybridItems<MODE, MULTI> src{jobs, njobs, lg2s, list, seeds, n0, size, jobs[0].S.nodes4, jobs[0].S.tris,
                                  jobs[0].S.node_count};
     item_loop<OFF32>(src, nitems, next_item, bits, stk + threadIdx.x, c);
     flush_counters(c, counters, false);
@@ -670,16 +679,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
 template <int MODE>
 void launch_spl_mode(bool multi, bool off32, uint32_t nb, hipStream_t st, const ShadowJob* jobs, uint32_t njobs,
                      uint32_t nitems, uint32_t lg2s, uint32_t* next_item, uint32_t* bits, const uint32_t* list,
-                     const uint32_t* list_count, uint32_t n0, uint32_t size, unsigned long long* counters) {
+                     const uint32_t* list_count, const uint32_t* seeds, uint32_t n0, uint32_t size,
+                     unsigned long long* counters) {
     if (multi)
         k_shadows_spl<MODE, true, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                               list_count, n0, size, counters);
+                                                               list_count, seeds, n0, size, counters);
     else if (off32)
         k_shadows_spl<MODE, false, true><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                               list_count, n0, size, counters);
+                                                               list_count, seeds, n0, size, counters);
     else
         k_shadows_spl<MODE, false, false><<<nb, BLOCK, 0, st>>>(jobs, njobs, nitems, lg2s, next_item, bits, list,
-                                                                list_count, n0, size, counters);
+                                                                list_count, seeds, n0, size, counters);
 }
 
 // shadow factor of every group from the bits of its samples
@@ -716,7 +726,8 @@ __global__ __launch_bounds__(256) void k_shadow_bits(const ShadowJob* __restrict
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict__ jobs, uint32_t njobs,
                                                      uint32_t ngroups, uint32_t lg2s, uint32_t probes,
-                                                     uint32_t* bits, unsigned long long* masks, uint32_t* counts) {
+                                                     uint32_t* bits, uint32_t* seeds, unsigned long long* masks,
+                                                     uint32_t* counts) {
     const uint32_t G = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = ITEM_SKIP;
     if (G < ngroups) {
@@ -728,6 +739,7 @@ __global__ __launch_bounds__(256) void k_group_kinds(const ShadowJob* __restrict
         const uint32_t li = rem >> 6;
         GroupRay gr;
         kind = group_kind(J, t, li, rem & 63u, gr);
+        if (kind == ITEM_TRACE) seeds[G] = tea(gr.y * F.width + gr.x, F.frame);
         if (kind == ITEM_LIT) {
             uint64_t b = (uint64_t)G << lg2s;
             const uint32_t lit = probes == 0u ? F.samples : (hard_light(F, li) ? 1u : min(F.samples, probes));
@@ -829,26 +841,28 @@ static uint32_t host_adaptive_next(uint32_t N, uint32_t S) {
 
 size_t shadows_spl_scratch(uint32_t ngroups, uint32_t lg2s, bool adaptive) {
     const size_t w = ((((size_t)ngroups << lg2s) + 31u) >> 5) * 4u;
-    // bits, the list of groups to trace and its count, the tail of list_groups
-    if (!adaptive) return w + (size_t)ngroups * 4u + 4u + group_list_tail_bytes(ngroups);
-    // bits, two lists of groups, their two counts on lines of their own, the
-    // tail of list_groups
-    return w + 2u * (size_t)ngroups * 4u + 2u * 256u + group_list_tail_bytes(ngroups);
+    // bits, the seeds, the list of groups to trace and its count, the tail of
+    // list_groups
+    if (!adaptive) return w + 2u * (size_t)ngroups * 4u + 4u + group_list_tail_bytes(ngroups);
+    // bits, the seeds, two lists of groups, their two counts on lines of their
+    // own, the tail of list_groups
+    return w + 3u * (size_t)ngroups * 4u + 2u * 256u + group_list_tail_bytes(ngroups);
 }
 
 // The groups that need rays, listed in group order into list / *count (the
 // others settled in bits, see k_group_kinds); tail: scratch for the chunk
 // masks, counts and tile totals (list_tail_bytes).
 static void list_groups(bool multi, const ShadowJob* jobs_dev, uint32_t njobs, uint32_t ngroups, uint32_t lg2s,
-                 uint32_t probes, uint32_t* bits, uint32_t* list, uint32_t* count, void* tail, hipStream_t st) {
+                        uint32_t probes, uint32_t* bits, uint32_t* seeds, uint32_t* list, uint32_t* count,
+                        void* tail, hipStream_t st) {
     const GroupListTail t = group_list_tail(tail, ngroups);
     const uint32_t gblocks = (ngroups + 255u) / 256u;
     if (multi)
-        k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, t.masks,
-                                                     t.counts);
+        k_group_kinds<true><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, seeds,
+                                                     t.masks, t.counts);
     else
-        k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, t.masks,
-                                                      t.counts);
+        k_group_kinds<false><<<gblocks, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, probes, bits, seeds,
+                                                      t.masks, t.counts);
     scan_and_scatter(ngroups, t, count, list, st);
 }
 
@@ -865,29 +879,30 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
     uint32_t* bits = scratch;
     k_reset_u32<<<1, 64, 0, st>>>(next_item);
     k_fill_bits<<<(words + 255u) / 256u, 256, 0, st>>>(bits, words);
+    uint32_t* seeds = bits + words;  // per group, written for the groups that need rays
     if (!adaptive) {
-        // scratch: bits, the list of groups to trace, its count, the tail
-        uint32_t* list = bits + words;
+        // scratch: bits, seeds, the list of groups to trace, its count, the tail
+        uint32_t* list = seeds + ngroups;
         uint32_t* count = list + ngroups;
-        list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 0u, bits, list, count, count + 1, st);
-        launch_spl_mode<3>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, 0u, 0u,
-                           counters);
+        list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 0u, bits, seeds, list, count, count + 1, st);
+        launch_spl_mode<3>(multi, off32, nb, st, jobs_dev, njobs, 0u, lg2s, next_item, bits, list, count, seeds, 0u,
+                           0u, counters);
         k_shadow_bits<<<(ngroups + 255u) / 256u, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits);
         return hipGetLastError();
     }
-    // scratch: bits (2^lg2s per group, every stage adds its own), lists A and
-    // B, their counts 64 words apart, the tail of list_groups. The probe
+    // scratch: bits (2^lg2s per group, every stage adds its own), seeds, lists
+    // A and B, their counts 64 words apart, the tail of list_groups. The probe
     // stage traces the groups listed in B (free until the first refinement
     // stage lists into it).
-    uint32_t* lists[2] = {bits + words, bits + words + ngroups};
-    uint32_t* counts[2] = {bits + words + 2u * (size_t)ngroups, bits + words + 2u * (size_t)ngroups + 64u};
+    uint32_t* lists[2] = {seeds + ngroups, seeds + 2u * (size_t)ngroups};
+    uint32_t* counts[2] = {seeds + 3u * (size_t)ngroups, seeds + 3u * (size_t)ngroups + 64u};
     const uint32_t S = 1u << lg2s;
     const uint32_t gb = (ngroups + 255u) / 256u;
     k_fill_bits<<<1, 64, 0, st>>>(counts[0], 1u);
-    list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 4u, bits, lists[1], counts[1], counts[1] + 64u, st);
+    list_groups(multi, jobs_dev, njobs, ngroups, lg2s, 4u, bits, seeds, lists[1], counts[1], counts[1] + 64u, st);
     const uint32_t pitems = ngroups << 2;
     launch_spl_mode<1>(multi, off32, min(resident_blocks, (pitems + BLOCK - 1) / BLOCK), st, jobs_dev, njobs, 0u,
-                       lg2s, next_item, bits, lists[1], counts[1], 0u, 4u, counters);
+                       lg2s, next_item, bits, lists[1], counts[1], seeds, 0u, 4u, counters);
     k_adaptive_decide<true><<<gb, 256, 0, st>>>(jobs_dev, njobs, ngroups, lg2s, bits, nullptr, nullptr, 4u,
                                                 lists[0], counts[0]);
     int cur = 0;
@@ -899,7 +914,8 @@ hipError_t launch_shadows_spl(const ShadowJob* jobs_dev, uint32_t njobs, uint32_
         k_fill_bits<<<1, 64, 0, st>>>(counts[cur ^ 1], 1u);
         const uint64_t want = (most + BLOCK - 1) / BLOCK;
         launch_spl_mode<2>(multi, off32, want < resident_blocks ? (uint32_t)want : resident_blocks, st,
-                           jobs_dev, njobs, 0u, lg2s, next_item, bits, lists[cur], counts[cur], N, size, counters);
+                           jobs_dev, njobs, 0u, lg2s, next_item, bits, lists[cur], counts[cur], seeds, N, size,
+                           counters);
         k_adaptive_decide<false><<<gb, 256, 0, st>>>(jobs_dev, njobs, 0u, lg2s, bits, lists[cur], counts[cur], Nn,
                                                      lists[cur ^ 1], counts[cur ^ 1]);
         cur ^= 1;
