@@ -28,7 +28,9 @@ def _val(b):
 
 @pytest.mark.parametrize("n,h,w,cin,cout,relu", [(1, 7, 13, 16, 25, False), (2, 16, 16, 48, 64, True),
                                                  (1, 9, 20, 80, 32, True), (1, 12, 8, 112, 48, False),
-                                                 (1, 8, 8, 128, 64, True), (3, 5, 3, 32, 4, False)])
+                                                 (1, 8, 8, 128, 64, True), (3, 5, 3, 32, 4, False),
+                                                 (1, 11, 37, 32, 32, True), (1, 9, 40, 32, 48, True),
+                                                 (2, 10, 17, 48, 48, False), (1, 13, 35, 64, 64, True)])
 def test_conv_forward(gpu_ctx, n, h, w, cin, cout, relu):
     from vulkanhybridraytracer_amd import denoise
     r = np.random.default_rng(cin * 1000 + cout)

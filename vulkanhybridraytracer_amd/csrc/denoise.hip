@@ -46,6 +46,53 @@ __host__ __device__ __forceinline__ size_t frag_index(uint32_t co, uint32_t k, u
     return ((size_t)((k >> 5) * (cout_pad >> 4) + (co >> 4)) * 64u + (co & 15u) + ((k & 31u) >> 3) * 16u) * 8u + (k & 7u);
 }
 
+// kernel prediction: out = sum_k softmax(logits)_k * noisy[clamp(p + off_k)]
+// over the 5x5 neighbourhood, off_k = (k % 5 - 2, k / 5 - 2)
+// lg: a pixel's first 28 logits (16-byte aligned), read as 16-byte chunks
+__device__ __forceinline__ void softmax25(const float* lg, float* w) {
+    const float4* l4 = reinterpret_cast<const float4*>(lg);
+    float v[28];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const float4 t = l4[q];
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
+    }
+    float m = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        w[k] = v[k];
+        m = fmaxf(m, w[k]);
+    }
+    float s = 0.0f;
+    for (int k = 0; k < 25; ++k) {
+        w[k] = expf(w[k] - m);
+        s += w[k];
+    }
+    float inv = 1.0f / s;
+    for (int k = 0; k < 25; ++k) w[k] *= inv;
+}
+__device__ __forceinline__ float4 tap_color(const float4* img, int x, int y, int k, uint32_t W, uint32_t H) {
+    int xx = min(max(x + k % 5 - 2, 0), (int)W - 1), yy = min(max(y + k / 5 - 2, 0), (int)H - 1);
+    return img[(size_t)yy * W + xx];
+}
+// the denoised colour of pixel (x, y) of a W x H frame from its logits
+__device__ __forceinline__ void kpcn_pixel(const float* lg, const float4* noisy, float4* out, int x, int y,
+                                           uint32_t W, uint32_t H) {
+    float w[25];
+    softmax25(lg, w);
+    float r = 0.f, g = 0.f, b = 0.f;
+    for (int k = 0; k < 25; ++k) {
+        float4 c = tap_color(noisy, x, y, k, W, H);
+        r += w[k] * c.x;
+        g += w[k] * c.y;
+        b += w[k] * c.z;
+    }
+    out[(size_t)y * W + x] = make_float4(r, g, b, 1.0f);
+}
+
 // ---- convolution (forward and input gradient) ----------------------------------
 constexpr int CONV_MT = 4;  // 16-pixel tiles per wave
 constexpr int CONV_NT = 4;  // 16-channel tiles per wave
@@ -174,12 +221,17 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t s_base) {
 // stride Cin + 8 elements against bank conflicts), so the 9 taps read their
 // fragments from LDS instead of re-reading global memory. Same K walk,
 // fragments and MFMA order per K step as k_conv. TWO: the halo's channels from
-// cin1 on come from in2 at half resolution (ConvArgs).
+// cin1 on come from in2 at half resolution (ConvArgs). KP: the kernel
+// prediction in the epilogue (ConvArgs::kp_out): the logits pass through LDS,
+// KP_STRIDE floats per pixel, where the halo tile was.
 constexpr int LT_H = 8, LT_W = 32, HALO_H = LT_H + 2, HALO_W = LT_W + 2;
 constexpr int HALO_BATCH = 8;
+constexpr uint32_t KP_STRIDE = 28;  // logits 0..27 of a pixel (25 used), 16-byte aligned rows
+constexpr size_t KP_LDS = 4u * 64u * KP_STRIDE * sizeof(float);
 
-template <int NS, bool TWO>
+template <int NS, bool TWO, bool KP = false>
 __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t s_base) {
+    static_assert(!KP || (NS == 2 && !TWO), "kernel prediction: one launch of 32 logits, one input");
     extern __shared__ __align__(16) __bf16 tile[];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint32_t ps = a.Cin + 8u;
@@ -303,7 +355,34 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     } else {
         kstep(wa, last{});
     }
-    conv_store<NS>(a, acc, pix, pok, s0, lane);
+    if constexpr (KP) {
+        // every wave is past its last halo read before the logits overwrite it
+        __syncthreads();
+        float* lg = reinterpret_cast<float*>(tile) + w * (64u * KP_STRIDE);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const uint32_t co = (s0 + s) * 16u + (lane >> 4) * 4u;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (co + r >= KP_STRIDE) continue;
+                const float b = (a.bias && co + r < a.Cout) ? a.bias[co + r] : 0.0f;
+#pragma unroll
+                for (int m = 0; m < CONV_MT; ++m) {
+                    float v = acc[m][s][r] + b;
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    lg[(m * 16u + (lane & 15u)) * KP_STRIDE + co + r] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // lane L: slot L of the wave = tile m = L / 16, column L % 16
+        const uint32_t m = lane >> 4;
+        const int y = y0 + (int)(2u * w + (m >> 1)), x = x0 + (int)((m & 1u) * 16u + (lane & 15u));
+        if (y < (int)a.kp_H && x < (int)a.kp_W)
+            kpcn_pixel(lg + lane * KP_STRIDE, a.kp_noisy, a.kp_out, x, y, a.kp_W, a.kp_H);
+    } else {
+        conv_store<NS>(a, acc, pix, pok, s0, lane);
+    }
 }
 
 // ---- weight gradient --------------------------------------------------------------
@@ -563,52 +642,11 @@ __global__ void k_features(FeatArgs a, uint32_t n) {
     o[1] = hi;
 }
 
-// kernel prediction: out = sum_k softmax(logits)_k * noisy[clamp(p + off_k)]
-// over the 5x5 neighbourhood, off_k = (k % 5 - 2, k / 5 - 2)
-// lg: a pixel's row of 32 logits (128-byte aligned), read as 16-byte chunks
-__device__ __forceinline__ void softmax25(const float* lg, float* w) {
-    const float4* l4 = reinterpret_cast<const float4*>(lg);
-    float v[28];
-#pragma unroll
-    for (int q = 0; q < 7; ++q) {
-        const float4 t = l4[q];
-        v[4 * q] = t.x;
-        v[4 * q + 1] = t.y;
-        v[4 * q + 2] = t.z;
-        v[4 * q + 3] = t.w;
-    }
-    float m = -INFINITY;
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-        w[k] = v[k];
-        m = fmaxf(m, w[k]);
-    }
-    float s = 0.0f;
-    for (int k = 0; k < 25; ++k) {
-        w[k] = expf(w[k] - m);
-        s += w[k];
-    }
-    float inv = 1.0f / s;
-    for (int k = 0; k < 25; ++k) w[k] *= inv;
-}
-__device__ __forceinline__ float4 tap_color(const float4* img, int x, int y, int k, uint32_t W, uint32_t H) {
-    int xx = min(max(x + k % 5 - 2, 0), (int)W - 1), yy = min(max(y + k / 5 - 2, 0), (int)H - 1);
-    return img[(size_t)yy * W + xx];
-}
 __global__ void k_kpcn(KpcnArgs a, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.W * a.H) return;
     int x = (int)(i % a.W), y = (int)(i / a.W);
-    float w[25];
-    softmax25(a.logits + ((size_t)n * a.Hp * a.Wp + (size_t)y * a.Wp + x) * 32, w);
-    float r = 0.f, g = 0.f, b = 0.f;
-    for (int k = 0; k < 25; ++k) {
-        float4 c = tap_color(a.noisy, x, y, k, a.W, a.H);
-        r += w[k] * c.x;
-        g += w[k] * c.y;
-        b += w[k] * c.z;
-    }
-    a.out[i] = make_float4(r, g, b, 1.0f);
+    kpcn_pixel(a.logits + ((size_t)n * a.Hp * a.Wp + (size_t)y * a.Wp + x) * 32, a.noisy, a.out, x, y, a.W, a.H);
 }
 // loss = sum over pixels and colour channels of |T(out) - T(target)|, T(v) =
 // max(v, 0) / (1 + max(v, 0)) (a.loss 0), or of (out - target)^2 / (target^2
@@ -724,6 +762,15 @@ hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
     if (a.in2 && (!lds_path || (a.cin1 & 7u) || (a.in2_C & 7u) || (a.H & 1u) || (a.W & 1u)))
         return hipErrorInvalidValue;  // two inputs: LDS path, 8-channel chunks, even sizes
     const uint32_t ns_total = a.Cout_pad >> 4;
+    if (a.kp_out) {
+        // fused kernel prediction: one image, 25..32 logits, the LDS path
+        if (!lds_path || a.in2 || a.N != 1u || ns_total != 2u || a.Cout < 25u || !a.kp_noisy || a.kp_W > a.W ||
+            a.kp_H > a.H)
+            return hipErrorInvalidValue;
+        const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
+        dn::k_conv_lds<2, false, true><<<tx * ty, 256, (lds > dn::KP_LDS ? lds : dn::KP_LDS), st>>>(a, tx, ty, 0u);
+        return hipGetLastError();
+    }
     const uint32_t full = ns_total / dn::CONV_NT, rem = ns_total % dn::CONV_NT;
     static_assert(dn::CONV_NT == 4, "remainder launches below cover 1..3 tiles");
     if (full) conv_launch<4>(a, full, 0u, lds_path, lds, st);

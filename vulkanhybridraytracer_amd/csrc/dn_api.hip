@@ -287,8 +287,10 @@ bool concat_of(int buf, Op& up) {
 // channels, dst_off of the upsampling), the pooling reads it there, and the
 // decoder layer that reads the concatenation takes its second part straight
 // from the upsampling's half-resolution source (two-input convolution). The
-// values are those of the written-out concatenation.
-hr_status forward(hr_denoiser* dn, const Arena& A) {
+// values are those of the written-out concatenation. With `image` (inference
+// of one image), the head layer's epilogue also applies the predicted kernels
+// and writes image->out; the logits are not written (kpcn_done: it did).
+hr_status forward(hr_denoiser* dn, const Arena& A, const hr_dn_image* image = nullptr, bool* kpcn_done = nullptr) {
     hr_context* ctx = dn->ctx;
     hipStream_t st = ctx->stream;
     const bool fused = !A.grads;
@@ -321,6 +323,13 @@ hr_status forward(hr_denoiser* dn, const Arena& A) {
             a.Cout = d.cout;
             a.Cout_pad = li.cout_pad;
             a.relu = op.layer != L_HEAD;
+            if (fused && image && op.dst == B_LOGITS && A.N == 1u) {
+                a.kp_noisy = reinterpret_cast<const float4*>(image->noisy);
+                a.kp_out = reinterpret_cast<float4*>(image->out);
+                a.kp_W = image->gbuffer.width;
+                a.kp_H = image->gbuffer.height;
+                if (kpcn_done) *kpcn_done = true;
+            }
             if (fused && op.dst != B_LOGITS && concat_of(op.dst, up)) a.out_C = up.dst_off;
             if (fused && concat_of(op.src, up)) {
                 if (op.src_off != 0u || up.dst_off + up.C != li.cin_pad) return fail(ctx, HR_E_INVALID, "concat layout");
@@ -678,8 +687,9 @@ hr_status hr_denoise(hr_denoiser* dn, const hr_dn_image* image, float* ms) {
     const Arena& A = dn->infer;
     if (ms) HR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     HR_HIP(ctx, dn_features(feat_args(A, *image), 0, ctx->stream));
-    HR_TRY(forward(dn, A));
-    HR_HIP(ctx, dn_kpcn(kpcn_args(A, *image, dn->loss), 0, ctx->stream));
+    bool kpcn_done = false;
+    HR_TRY(forward(dn, A, image, &kpcn_done));
+    if (!kpcn_done) HR_HIP(ctx, dn_kpcn(kpcn_args(A, *image, dn->loss), 0, ctx->stream));
     if (ms) {
         HR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
         HR_HIP(ctx, hipEventSynchronize(ctx->ev[1]));

@@ -295,6 +295,13 @@ struct ConvArgs {
     // a nearest 2x upsampling, without either written out
     const __bf16* in2;
     uint32_t in2_C, cin1;
+    // kernel prediction fused into the epilogue (kp_out not null; the LDS
+    // path, N = 1, 2 output tiles): no logits are written; every pixel of the
+    // kp_W x kp_H frame gets out = the 5x5 softmax-weighted sum of kp_noisy
+    // its logits (bias added) predict, as dn_kpcn computes it
+    const float4* kp_noisy;
+    float4* kp_out;
+    uint32_t kp_W, kp_H;
 };
 struct WgradArgs {
     const __bf16* dout;    // gradient of the layer output (after the ReLU mask)
