@@ -229,7 +229,7 @@ constexpr int HALO_BATCH = 8;
 constexpr uint32_t KP_STRIDE = 28;  // logits 0..27 of a pixel (25 used), 16-byte aligned rows
 constexpr size_t KP_LDS = 4u * 64u * KP_STRIDE * sizeof(float);
 
-template <int NS, bool TWO, bool KP = false>
+template <int NS, bool TWO, bool KP = false, bool DEEP = (NS > 2 || TWO)>
 __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, uint32_t tiles_y, uint32_t s_base) {
     static_assert(!KP || (NS == 2 && !TWO), "kernel prediction: one launch of 32 logits, one input");
     extern __shared__ __align__(16) __bf16 tile[];
@@ -327,8 +327,10 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
             ++tap;
         }
     };
-    // Weight fragments one K step ahead in two buffers used in turn (no copy
-    // between them, so waiting for one does not wait for the other). The
+    // Weight fragments one K step ahead in two buffers used in turn, or two
+    // steps ahead in three (DEEP: wider tiles, two-input layers and small
+    // grids, where few waves per SIMD hide the load latency); no copy between
+    // them, so waiting for one does not wait for the others. The
     // prefetch index is clamped to the last step: always in bounds, and the
     // value loaded past the end is not used.
     auto wload = [&](bf16x8 (&wv)[NS], uint32_t k) {
@@ -338,22 +340,47 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     };
     using full = std::false_type;  // every K of the step is a real tap
     using last = std::true_type;   // the last step
-    bf16x8 wa[NS], wb[NS];
-    wload(wa, 0u);
     const uint32_t nmain = nk - 1u;
     uint32_t k = 0;
-    for (; k + 2u <= nmain; k += 2u) {
-        wload(wb, k + 1u);
-        kstep(wa, full{});
-        wload(wa, k + 2u);
-        kstep(wb, full{});
-    }
-    if (k < nmain) {  // one full step left before the last
-        wload(wb, k + 1u);
-        kstep(wa, full{});
-        kstep(wb, last{});
+    bf16x8 wa[NS], wb[NS];
+    wload(wa, 0u);
+    if constexpr (DEEP) {
+        bf16x8 wc[NS];
+        wload(wb, 1u);
+        for (; k + 3u <= nmain; k += 3u) {
+            wload(wc, k + 2u);
+            kstep(wa, full{});
+            wload(wa, k + 3u);
+            kstep(wb, full{});
+            wload(wb, k + 4u);
+            kstep(wc, full{});
+        }
+        // 0, 1 or 2 full steps left before the last (wa holds step k, wb step k + 1)
+        if (k + 2u == nmain) {
+            wload(wc, k + 2u);
+            kstep(wa, full{});
+            kstep(wb, full{});
+            kstep(wc, last{});
+        } else if (k + 1u == nmain) {
+            kstep(wa, full{});
+            kstep(wb, last{});
+        } else {
+            kstep(wa, last{});
+        }
     } else {
-        kstep(wa, last{});
+        for (; k + 2u <= nmain; k += 2u) {
+            wload(wb, k + 1u);
+            kstep(wa, full{});
+            wload(wa, k + 2u);
+            kstep(wb, full{});
+        }
+        if (k < nmain) {  // one full step left before the last
+            wload(wb, k + 1u);
+            kstep(wa, full{});
+            kstep(wb, last{});
+        } else {
+            kstep(wa, last{});
+        }
     }
     if constexpr (KP) {
         // every wave is past its last halo read before the logits overwrite it
@@ -770,6 +797,16 @@ hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
         const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
         dn::k_conv_lds<2, false, true><<<tx * ty, 256, (lds > dn::KP_LDS ? lds : dn::KP_LDS), st>>>(a, tx, ty, 0u);
         return hipGetLastError();
+    }
+    if (lds_path && !a.in2 && ns_total >= 4u && (ns_total & 1u) == 0u) {
+        // small grids (the 1/8 and 1/16 resolution layers: 36-136 tiles for
+        // 256 CUs): two 16-channel tiles per wave over twice the workgroups
+        // (not for two inputs, whose halo copy costs more than it gains)
+        const uint32_t tx = (a.W + dn::LT_W - 1) / dn::LT_W, ty = (a.H + dn::LT_H - 1) / dn::LT_H;
+        if (a.N * tx * ty < 256u) {
+            dn::k_conv_lds<2, false, false, true><<<dim3(a.N * tx * ty, ns_total / 2u), 256, lds, st>>>(a, tx, ty, 0u);
+            return hipGetLastError();
+        }
     }
     const uint32_t full = ns_total / dn::CONV_NT, rem = ns_total % dn::CONV_NT;
     static_assert(dn::CONV_NT == 4, "remainder launches below cover 1..3 tiles");
