@@ -221,10 +221,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, uint32_t s_base) {
 // stride Cin + 8 elements against bank conflicts), so the 9 taps read their
 // fragments from LDS instead of re-reading global memory. Same K walk,
 // fragments and MFMA order per K step as k_conv. TWO: the halo's channels from
-// cin1 on come from in2 at half resolution (ConvArgs). KP: the kernel
+// cin1 on come from in2 at half resolution (ConvArgs); LDS holds them once per
+// half-resolution pixel (a 6 x 18 halo after the 10 x 34 one), and a tap reads
+// the half-resolution pixel under its full-resolution one. KP: the kernel
 // prediction in the epilogue (ConvArgs::kp_out): the logits pass through LDS,
 // KP_STRIDE floats per pixel, where the halo tile was.
 constexpr int LT_H = 8, LT_W = 32, HALO_H = LT_H + 2, HALO_W = LT_W + 2;
+constexpr int HB_H = LT_H / 2 + 2, HB_W = LT_W / 2 + 2;  // half-resolution halo of a two-input layer
 constexpr int HALO_BATCH = 8;
 constexpr uint32_t KP_STRIDE = 28;  // logits 0..27 of a pixel (25 used), 16-byte aligned rows
 constexpr size_t KP_LDS = 4u * 64u * KP_STRIDE * sizeof(float);
@@ -234,7 +237,12 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     static_assert(!KP || (NS == 2 && !TWO), "kernel prediction: one launch of 32 logits, one input");
     extern __shared__ __align__(16) __bf16 tile[];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    const uint32_t ps = a.Cin + 8u;
+    // TWO: the halo of in (cin1 channels, stride ps) is followed at
+    // base2 by the half-resolution halo of in2 (HB_H x HB_W pixels, Cin - cin1
+    // channels, stride ps2)
+    const uint32_t cin1 = TWO ? a.cin1 : a.Cin;
+    const uint32_t ps = cin1 + 8u, ps2 = a.Cin - cin1 + 8u;
+    const uint32_t base2 = (uint32_t)(HALO_H * HALO_W) * ps;
     uint32_t b = blockIdx.x;
     const uint32_t tx = b % tiles_x;
     b /= tiles_x;
@@ -242,8 +250,9 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     const uint32_t n = b / tiles_y;
     const int x0 = (int)(tx * LT_W), y0 = (int)(ty * LT_H);
     const int H = (int)a.H, W = (int)a.W;
-    const uint32_t cpp = a.Cin >> 3;  // 16-byte chunks per pixel
-    const uint32_t total = (uint32_t)(HALO_H * HALO_W) * cpp;
+    const uint32_t cpp = cin1 >> 3, cpp2 = (a.Cin - cin1) >> 3;  // 16-byte chunks per pixel
+    const uint32_t total1 = (uint32_t)(HALO_H * HALO_W) * cpp;
+    const uint32_t total = total1 + (TWO ? (uint32_t)(HB_H * HB_W) * cpp2 : 0u);
     const bf16x8 zero = {};
     // HALO_BATCH chunks per thread in flight before they are written to LDS,
     // so the global latency is paid once per batch, not once per chunk
@@ -255,21 +264,24 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
             const uint32_t i = i0 + (uint32_t)j * 256u;
             v[j] = zero;
             dst[j] = ~0u;
-            if (i < total) {
+            if (i < total1) {
                 const uint32_t hp = i / cpp, ch = i - hp * cpp;
                 const int y = y0 + (int)(hp / HALO_W) - 1, x = x0 + (int)(hp % HALO_W) - 1;
-                if (y >= 0 && y < H && x >= 0 && x < W) {
-                    if (!TWO || ch < (a.cin1 >> 3))
-                        v[j] = *reinterpret_cast<const bf16x8*>(
-                            a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C + a.in_off +
-                            ch * 8u);
-                    else
-                        v[j] = *reinterpret_cast<const bf16x8*>(
-                            a.in2 + ((size_t)(n * (a.H >> 1) + ((uint32_t)y >> 1)) * (a.W >> 1) + ((uint32_t)x >> 1)) *
-                                        a.in2_C +
-                            (ch * 8u - a.cin1));
-                }
+                if (y >= 0 && y < H && x >= 0 && x < W)
+                    v[j] = *reinterpret_cast<const bf16x8*>(
+                        a.in + ((size_t)(n * a.H + (uint32_t)y) * a.W + (uint32_t)x) * a.in_C + a.in_off + ch * 8u);
                 dst[j] = hp * ps + ch * 8u;
+            } else if (TWO && i < total) {
+                // half-resolution pixel (y0 / 2 - 1 + row, x0 / 2 - 1 + col):
+                // rows and columns of in2 that the nearest 2x upsampling of
+                // the halo reads; zero outside in2's H / 2 x W / 2
+                const uint32_t i2 = i - total1, hq = i2 / cpp2, ch = i2 - hq * cpp2;
+                const int Y = (y0 >> 1) - 1 + (int)(hq / HB_W), X = (x0 >> 1) - 1 + (int)(hq % HB_W);
+                if (Y >= 0 && Y < (H >> 1) && X >= 0 && X < (W >> 1))
+                    v[j] = *reinterpret_cast<const bf16x8*>(
+                        a.in2 + ((size_t)(n * (a.H >> 1) + (uint32_t)Y) * (a.W >> 1) + (uint32_t)X) * a.in2_C +
+                        ch * 8u);
+                dst[j] = base2 + hq * ps2 + ch * 8u;
             }
         }
 #pragma unroll
@@ -309,12 +321,28 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a, uint32_t tiles_x, 
     auto kstep = [&](const bf16x8 (&wv)[NS], auto last) {
         const bool kval = !decltype(last)::value || tap < 9u;
         const uint32_t tc = kval ? tap : 4u;  // padding K: any in-tile address, value unused
-        const int doff = (((int)(tc / 3u) - 1) * HALO_W + (int)(tc % 3u) - 1) * (int)ps + (int)ci;
+        const int dy = (int)(tc / 3u) - 1, dx = (int)(tc % 3u) - 1;
+        const int doff = (dy * HALO_W + dx) * (int)ps + (int)ci;
         bf16x8 av[CONV_MT];
+        if constexpr (TWO) {
+            // channels from cin1 on: tap pixel (ry + dy, rx + dx) of the tile
+            // reads half-resolution halo pixel ((ry + dy + 2) / 2, (rx + dx + 2) / 2)
+            const bool second = ci >= cin1;
+            const int rw = (int)(2u * w) + dy + 2, cl = (int)(lane & 15u) + dx + 2;
+            const int b2 = (int)base2 + (int)ci - (int)cin1 + (cl >> 1) * (int)ps2;
+            const int r0 = (rw >> 1) * HB_W * (int)ps2, r1 = ((rw + 1) >> 1) * HB_W * (int)ps2;
 #pragma unroll
-        for (int m = 0; m < CONV_MT; ++m) {
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (rbase[m] + doff));
-            av[m] = kval ? v : zero;
+            for (int m = 0; m < CONV_MT; ++m) {
+                const int o2 = b2 + ((m >> 1) ? r1 : r0) + (m & 1) * 8 * (int)ps2;
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (second ? o2 : rbase[m] + doff));
+                av[m] = kval ? v : zero;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < CONV_MT; ++m) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8*>(tile + (rbase[m] + doff));
+                av[m] = kval ? v : zero;
+            }
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -784,7 +812,13 @@ static void conv_launch(const ConvArgs& a, uint32_t ny, uint32_t s_base, bool ld
 hipError_t dn_conv(const ConvArgs& a, hipStream_t st) {
     // LDS-staged halo when the tile fits in 96 KiB (up to 128 input channels;
     // gfx950 has 160 KiB per CU) and 16-byte chunks stay aligned
-    const size_t lds = (size_t)dn::HALO_H * dn::HALO_W * (a.Cin + 8u) * sizeof(__bf16);
+    // (two inputs: the halo of in's cin1 channels, then in2's half-resolution
+    // halo of the other Cin - cin1)
+    if (a.in2 && (a.cin1 == 0u || a.cin1 >= a.Cin)) return hipErrorInvalidValue;
+    const uint32_t cin1 = a.in2 ? a.cin1 : a.Cin;
+    const size_t lds = ((size_t)dn::HALO_H * dn::HALO_W * (cin1 + 8u) +
+                        (a.in2 ? (size_t)dn::HB_H * dn::HB_W * (a.Cin - cin1 + 8u) : 0u)) *
+                       sizeof(__bf16);
     const bool lds_path = lds <= 98304u && (a.Cin & 7u) == 0 && (a.in_C & 7u) == 0 && (a.in_off & 7u) == 0;
     if (a.in2 && (!lds_path || (a.cin1 & 7u) || (a.in2_C & 7u) || (a.H & 1u) || (a.W & 1u)))
         return hipErrorInvalidValue;  // two inputs: LDS path, 8-channel chunks, even sizes
