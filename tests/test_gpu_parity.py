@@ -274,12 +274,12 @@ def _binary_children(nodes, i):
 @pytest.mark.parametrize("leaf_max", [1, 4])
 def test_wide_nodes_match_binary(gpu_ctx, oracle_lib, leaf_max):
     """the 4-wide nodes of the timed kernels: stored densely (the root first,
-    then in the order of the binary nodes they are rooted at, a third to a
-    half as many as binary nodes); from the root they reach exactly the binary
+    then in the order of the binary nodes they are rooted at, fewer than 0.6
+    as many as binary nodes); from the root they reach exactly the binary
     tree's leaves, each child box equal to the binary box of the same subtree,
-    empty slots with inverted boxes, and every wide node is the greedy
-    largest-area collapse of its binary node. A refit keeps the wide topology
-    and the boxes follow the refitted binary nodes."""
+    empty slots with inverted boxes, and every wide node is the collapse of
+    its binary node that minimises the summed wide-node box area. A refit
+    keeps the wide topology and the boxes follow the refitted binary nodes."""
     sc = scenes.make_terrain_scene(seed=2, width=32, height=18, quads=64, instances=24, soup_tris=256)
     hr = _hr()
     hs = hr.HybridScene(gpu_ctx, sc, leaf_max=leaf_max)
@@ -345,31 +345,69 @@ def check_wide_nodes(hs, LEAF, check_collapse):
     if not check_collapse:
         return
 
-    # the collapse rule itself: from the binary node's two children, twice open
-    # the internal child of largest area (x y + y z) + z x of the extents in
-    # float32, the first one on ties; the opened child's first child takes its
-    # slot and the second is appended
+    # the collapse rule itself, restated in float32: F(n, k) (k = 1..4), the
+    # least summed box area of the wide nodes below a cut of n's subtree into
+    # at most k slots, bottom-up (F = 0 for a leaf; F(n, 1) = area(n) +
+    # S(n, 4), F(n, k) = min(F(n, 1), S(n, k)), S(n, k) = min over k1 of
+    # F(left, k1) + F(right, k - k1), the first k1 on ties); then from each
+    # wide root its budget of 4 split by the same minima, a slot opened when
+    # S(c, k) < F(c, 1), its first child taking the slot and the second
+    # appended
     f32 = np.float32
 
     def area(lo, hi):
         e = [max(f32(hi[a]) - f32(lo[a]), f32(0)) for a in range(3)]
         return (e[0] * e[1] + e[1] * e[2]) + e[2] * e[0]
 
+    F = {}
+    zero = [f32(0)] * 4
+
+    def child_f(i, side):
+        code = _binary_children(bn, i)[side][0]
+        return zero if code & LEAF else F[code]
+
+    def split(l, r, k):
+        best, k1 = f32(l[0] + r[k - 2]), 1
+        for j in range(2, k):
+            c = f32(l[j - 1] + r[k - j - 1])
+            if c < best:
+                best, k1 = c, j
+        return best, k1
+
+    order, todo = [], [0]
+    while todo:
+        i = todo.pop()
+        order.append(i)
+        todo += [c for c, _, _ in _binary_children(bn, i) if not c & LEAF]
+    if len(bn) > 1:
+        for i in reversed(order):
+            (_, lo0, hi0), (_, lo1, hi1) = _binary_children(bn, i)
+            lo = [min(f32(lo0[a]), f32(lo1[a])) for a in range(3)]
+            hi = [max(f32(hi0[a]), f32(hi1[a])) for a in range(3)]
+            l, r = child_f(i, 0), child_f(i, 1)
+            s2, s3, s4 = (split(l, r, k)[0] for k in (2, 3, 4))
+            f1 = f32(area(lo, hi) + s4)
+            F[i] = [f1, min(f1, s2), min(f1, s3), min(f1, s4)]
+
     stack, roots = [0], 0
     while stack:
         i = stack.pop()
         roots += 1
-        ch = _binary_children(bn, i)
-        for _ in range(2):
-            best, best_a = -1, f32(-1)
-            for k, (code, lo, hi) in enumerate(ch):
-                if not code & LEAF and area(lo, hi) > best_a:
-                    best, best_a = k, area(lo, hi)
-            if best < 0:
-                break
-            g = _binary_children(bn, ch[best][0])
-            ch[best] = g[0]
-            ch.append(g[1])
+        bc = _binary_children(bn, i)
+        k1 = split(child_f(i, 0), child_f(i, 1), 4)[1]
+        slots = [[k1, bc[0]], [4 - k1, bc[1]]]
+        q = 0
+        while q < len(slots):
+            kb, (code, _, _) = slots[q]
+            if kb > 1 and not code & LEAF and len(slots) < 4:
+                sc, j1 = split(child_f(code, 0), child_f(code, 1), kb)
+                if not F[code][0] <= sc:
+                    cc = _binary_children(bn, code)
+                    slots.append([kb - j1, cc[1]])
+                    slots[q] = [j1, cc[0]]
+                    continue
+            q += 1
+        ch = [c for _, c in slots]
         got = wide(i)
         for k in range(4):
             if k < len(ch):
